@@ -1,0 +1,84 @@
+// POD descriptors shared by host code and the HIP kernels.
+#pragma once
+#include <cstdint>
+
+namespace fsgc {
+
+enum MotifClass : int32_t { kVertex = 0, kEdge = 1, kWedge = 2, kTriangle = 3, kGeneral = 4 };
+
+// Variants inside a class (which reference kernel's semantics apply).
+enum : int32_t {
+  kVertexCount = 0,    // anchorsets == 0 -> n           (FC:2175-2177)
+  kVertexEdges = 1,    // fcnt[0] == 1 -> m / 2          (FC:2180-2182)
+  kVertexStars = 2,    // sum_u C(d_u, t)                (FC:2152-2169)
+  kEdgeTriangle = 0,   // triangles w < v < u            (FC:1864-1894)
+  kEdgeTailed = 1,     // tailed triangles               (FC:1931-1966)
+  kEdgeGeneral = 2,    // closed form per edge           (FC:1999-2051)
+  kWedgeSpecial = 0,   // centre without fringes         (FC:1409-1477)
+  kWedgeGeneral = 1,   // fc3 with rotations             (FC:1513-1650)
+  kTriClique = 0,      // 4-cliques                      (FC:774-809)
+  kTriTailedClique = 1,// tailed 4-cliques               (FC:891-931)
+  kTriGeneral = 2,     // fc3 with rotations             (FC:1027-1062)
+};
+
+// Result flags (also in include/fsgc.h).
+enum : uint32_t {
+  kOvf64 = 1u << 0,
+  kOvf128 = 1u << 1,
+  kRefNckInexact = 1u << 2,
+  kAutRemainder = 1u << 3,
+  kRefAutTrunc = 1u << 4,
+};
+
+constexpr int kMaxPlanLevels = 96;
+
+// Device-side u128 accumulator of one job (one motif on one graph).
+struct Accum {
+  uint64_t lo;
+  uint64_t hi;
+  uint64_t matches;
+  uint32_t flags;
+  uint32_t pad;
+};
+
+struct JobDesc {
+  int32_t cls;
+  int32_t kind;
+  const int32_t* idx;
+  const int32_t* adj;
+  int32_t n;
+  int32_t m;
+  int32_t smod;   // slice: only roots u with u % smod == srem (smod <= 1: all)
+  int32_t srem;
+  // scalar parameters; meaning per class:
+  //   vertex:   p[0] = tails
+  //   edge:     p[0] f_u, p[1] f_uv, p[2] f_v, p[3] lim1, p[4] lim2
+  //   wedge:    p[0] min_u, p[1] lim, p[2] sum1, p[3] sum2, p[4] sum3,
+  //             special: p[5] f_v, p[6] f_vw, p[7] f_w, p[8] smin_v, p[9] smin_w, p[10] slim
+  //   triangle: p[0] sum1, p[1] sum2, p[2] sum3, p[3] tails
+  //   general:  p[0] nodes, p[1] startdeg, p[2..6] destdeg, p[7..11] list, p[12] nonanch, p[13] top
+  int32_t p[16];
+  int32_t rot;
+  int32_t fr[6][8];
+  // general core: fringe counts by anchor bitmask and the functionX evaluation plan
+  int32_t fcnt_by_mask[64];
+  int32_t nlev;                    // number of (anchor set, region) levels
+  int8_t lev_set[kMaxPlanLevels];  // anchor bitmask of the level
+  int8_t lev_reg[kMaxPlanLevels];  // region bitmask of the level
+  int8_t lev_last[kMaxPlanLevels]; // 1 if the region is the last (top) one for this set
+  int8_t lev_final[kMaxPlanLevels];// 1 if this is the last level of the whole plan
+  Accum* out;
+};
+
+// One unit of the device work queue: a range of roots of one job.
+// Roots are vertices for the vertex class and CSR positions otherwise.
+struct WorkItem {
+  int32_t job;
+  int32_t u0;     // owner of position `begin` (edge-rooted classes)
+  int32_t u1;     // owner of position `end - 1`
+  int32_t pad;
+  int64_t begin;
+  int64_t end;
+};
+
+}  // namespace fsgc

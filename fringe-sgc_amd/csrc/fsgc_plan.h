@@ -1,0 +1,53 @@
+// Motif planning (host) and the device driver interface.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "fsgc_host.h"
+#include "fsgc_job.h"
+
+namespace fsgc {
+
+int classify(const MatchingOrder& mo);
+int validate_mo(const MatchingOrder& mo, std::string* err);
+int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err);
+int general_quirk_factor(const MatchingOrder& mo);
+int triangle_rotations(const int f[8], int fr[6][8]);
+int wedge_rotations(const int f[8], int fr[6][8]);
+HostGraph motif_graph(const MatchingOrder& mo);
+
+// Launch parameters of the persistent counting kernel.
+struct Tune {
+  int block = 256;        // threads per workgroup (multiple of 64)
+  int grid = 0;           // workgroups (0: 8 per CU)
+  int chunk = 256;        // roots (CSR positions or vertices) per work item
+  int hub_degree = 4096;  // roots of degree above this are split into single-root items
+  int lds_cap = 2048;     // longest list staged in LDS by the wedge/triangle paths
+};
+
+// A graph resident in device memory (owned by the library).
+struct DeviceGraph {
+  int32_t n = 0;
+  int64_t m = 0;
+  int32_t maxdeg = 0;
+  int32_t* idx = nullptr;   // n + 1
+  int32_t* adj = nullptr;   // m
+  std::vector<int32_t> h_idx;  // host copy of nindex (used to cut work items)
+};
+
+struct JobResult {
+  u128 sum = 0;
+  uint64_t matches = 0;
+  uint32_t flags = 0;
+};
+
+// kernels.hip
+int device_init(int device, std::string* err);
+int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err);
+void device_graph_free(DeviceGraph* dg);
+// Runs the given jobs (each bound to a graph) in one persistent launch on `stream`
+// and returns one result per job. kernel_ms receives the device time of the launch.
+int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
+                    void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
+
+}  // namespace fsgc

@@ -1,0 +1,221 @@
+// Host-side motif planning: from a matching order, decide the core class and the
+// reference kernel semantics that apply, and precompute every per-motif constant the
+// kernels need (fringe vectors, rotations, degree thresholds, the functionX plan).
+#include "fsgc_plan.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fsgc {
+
+namespace {
+enum { iu = 1, iv = 2, iw = 4, iuv = 3, iuw = 5, ivw = 6, iuvw = 7 };  // FC:61-67
+
+void put_row(int fr[6][8], int r, int uvw, int uv, int uw, int vw, int u, int v, int w)
+{
+  fr[r][0] = 0; fr[r][iuvw] = uvw; fr[r][iuv] = uv; fr[r][iuw] = uw; fr[r][ivw] = vw;
+  fr[r][iu] = u; fr[r][iv] = v; fr[r][iw] = w;
+}
+
+// rotate: (uv, uw, vw, u, v, w) <- (vw, uv, uw, v, w, u) and its square
+void rotations_from(int fr[6][8], int& p, int q)
+{
+  put_row(fr, p++, fr[q][iuvw], fr[q][ivw], fr[q][iuv], fr[q][iuw], fr[q][iv], fr[q][iw], fr[q][iu]);
+  put_row(fr, p++, fr[q][iuvw], fr[q][iuw], fr[q][ivw], fr[q][iuv], fr[q][iw], fr[q][iu], fr[q][iv]);
+}
+}  // namespace
+
+// FC:1291-1317: the distinct relabelings of a triangle's fringe vector (1, 3 or 6).
+int triangle_rotations(const int f[8], int fr[6][8])
+{
+  int p = 0;
+  put_row(fr, p++, f[iuvw], f[iuv], f[iuw], f[ivw], f[iu], f[iv], f[iw]);
+  if ((f[iu] != f[iv]) || (f[iu] != f[iw]) || (f[iuv] != f[iuw]) || (f[iuv] != f[ivw])) {
+    rotations_from(fr, p, 0);
+    if (((f[iv] != f[iw]) || (f[iuv] != f[iuw])) && ((f[iu] != f[iw]) || (f[iuv] != f[ivw])) &&
+        ((f[iu] != f[iv]) || (f[iuw] != f[ivw]))) {
+      put_row(fr, p++, f[iuvw], f[iuw], f[iuv], f[ivw], f[iu], f[iw], f[iv]);
+      rotations_from(fr, p, p - 1);
+    }
+  }
+  return p;
+}
+
+// FC:1754-1780: a wedge always has the 3 rotations (used for triangles) and the flip
+// v <-> w when the endpoints' fringe vectors differ.
+int wedge_rotations(const int f[8], int fr[6][8])
+{
+  int p = 0;
+  put_row(fr, p++, f[iuvw], f[iuv], f[iuw], f[ivw], f[iu], f[iv], f[iw]);
+  rotations_from(fr, p, 0);
+  if ((f[iv] != f[iw]) || (f[iuv] != f[iuw])) {
+    put_row(fr, p++, f[iuvw], f[iuw], f[iuv], f[ivw], f[iu], f[iw], f[iv]);
+    rotations_from(fr, p, p - 1);
+  }
+  return p;
+}
+
+int classify(const MatchingOrder& mo)
+{
+  if (mo.nodes == 1) return kVertex;
+  if (mo.nodes == 2) return kEdge;
+  if (mo.nodes == 3) return (__builtin_popcount(mo.list[1]) == 1) ? kWedge : kTriangle;  // FC:2250
+  return kGeneral;
+}
+
+int validate_mo(const MatchingOrder& mo, std::string* err)
+{
+  if (mo.nodes < 1 || mo.nodes > kMaxCore) { if (err) *err = "matching order: core size must be 1..6"; return -22; }
+  if (mo.anchorsets > 63) { if (err) *err = "matching order: too many anchor sets"; return -22; }
+  const int full = (1 << mo.nodes) - 1;
+  int total = mo.nodes;
+  for (int i = 0; i < mo.anchorsets; i++) {
+    if (mo.anch[i] == 0 || (mo.anch[i] & ~full)) { if (err) *err = "matching order: anchor set outside the core"; return -22; }
+    if (i > 0 && mo.anch[i] <= mo.anch[i - 1]) { if (err) *err = "matching order: anchor sets not ascending"; return -22; }
+    total += mo.fcnt[i];
+  }
+  if (total > kMaxPattern) { if (err) *err = "matching order: more than 32 motif vertices"; return -22; }
+  for (int i = 0; i + 1 < mo.nodes; i++) {
+    if (mo.list[i] == 0 || (mo.list[i] >> (i + 1))) { if (err) *err = "matching order: invalid connection list"; return -22; }
+  }
+  return 0;
+}
+
+// The reference's functionX repeats its first level fcnt[last] + 1 times when the last
+// anchor set is the whole anchor universe and there is more than one anchor set
+// (FC:428 computes `a` without the idx == top test of FC:480). Every general-core sum
+// (data graph and motif graph) is scaled by that constant.
+int general_quirk_factor(const MatchingOrder& mo)
+{
+  if (mo.nodes < 4 || mo.anchorsets < 2) return 1;
+  int anchors = 0;
+  for (int i = 0; i < mo.anchorsets; i++) anchors |= mo.anch[i];
+  return (mo.anch[mo.anchorsets - 1] == anchors) ? mo.fcnt[mo.anchorsets - 1] + 1 : 1;
+}
+
+int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
+{
+  int rc = validate_mo(mo, err);
+  if (rc) return rc;
+  memset(j, 0, sizeof(*j));
+  j->cls = classify(mo);
+  j->smod = 1;
+  int f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  switch (j->cls) {
+    case kVertex:  // FC:2172-2191
+      if (mo.anchorsets == 0) j->kind = kVertexCount;
+      else if (mo.fcnt[0] == 1) j->kind = kVertexEdges;
+      else { j->kind = kVertexStars; j->p[0] = mo.fcnt[0]; }
+      break;
+    case kEdge: {  // FC:2106-2123
+      for (int i = 0; i < mo.anchorsets; i++) f[mo.anch[i] & 3] = mo.fcnt[i];
+      const int f_u = std::min(f[1], f[2]), f_uv = f[3], f_v = std::max(f[1], f[2]);
+      const bool tri = (f_uv == 1) && ((f_u == 0) || (f_v == 0));
+      j->kind = tri ? ((f_u == 0 && f_v == 0) ? kEdgeTriangle : kEdgeTailed) : kEdgeGeneral;
+      j->p[0] = f_u; j->p[1] = f_uv; j->p[2] = f_v;
+      j->p[3] = tri ? 1 : f_u + f_uv;
+      j->p[4] = tri ? 1 : f_u + f_uv + f_v;
+      break;
+    }
+    case kWedge: {  // FC:1783-1820
+      for (int i = 0; i < mo.anchorsets; i++) f[mo.anch[i] & 7] = mo.fcnt[i];
+      if (mo.list[1] == 2) { std::swap(f[iu], f[iv]); std::swap(f[iuw], f[ivw]); }
+      const int min_u = f[iuvw] + f[iuv] + f[iuw] + f[iu] + 2;
+      const int min_v = f[iuvw] + f[iuv] + f[ivw] + f[iv] + 1;
+      const int min_w = f[iuvw] + f[iuw] + f[ivw] + f[iw] + 1;
+      const int sum3 = min_u + min_v + min_w;
+      const int sum2 = sum3 - std::max(std::max(min_u, min_v), min_w);
+      const int sum1 = std::min(std::min(min_u, min_v), min_w);
+      const int lim = std::min(min_v, min_w);
+      j->p[0] = min_u; j->p[1] = lim; j->p[2] = sum1; j->p[3] = sum2; j->p[4] = sum3;
+      if ((f[iuvw] == 0) && (f[iuv] == 0) && (f[iuw] == 0) && (f[iu] == 0)) {
+        j->kind = kWedgeSpecial;  // FC:1409-1415
+        j->p[5] = f[iv]; j->p[6] = f[ivw]; j->p[7] = f[iw];
+        j->p[8] = f[iv] + f[ivw] + 1;
+        j->p[9] = f[ivw] + f[iw] + 1;
+        j->p[10] = std::min(j->p[8], j->p[9]);
+      } else {
+        j->kind = kWedgeGeneral;
+        j->rot = wedge_rotations(f, j->fr);
+      }
+      break;
+    }
+    case kTriangle: {  // FC:1320-1368
+      for (int i = 0; i < mo.anchorsets; i++) f[mo.anch[i] & 7] = mo.fcnt[i];
+      const int min_u = f[iuvw] + f[iuv] + f[iuw] + f[iu];
+      const int min_v = f[iuvw] + f[iuv] + f[ivw] + f[iv];
+      const int min_w = f[iuvw] + f[iuw] + f[ivw] + f[iw];
+      const int sum3 = min_u + min_v + min_w + 6;
+      const int sum2 = sum3 - (std::max(std::max(min_u, min_v), min_w) + 2);
+      const int sum1 = std::min(std::min(min_u, min_v), min_w) + 2;
+      j->p[0] = sum1; j->p[1] = sum2; j->p[2] = sum3;
+      j->p[3] = f[iu] + f[iv] + f[iw];
+      j->rot = triangle_rotations(f, j->fr);
+      if ((f[iuvw] == 1) && (f[iuv] + f[iuw] + f[ivw] == 0) && ((f[iu] == 0) + (f[iv] == 0) + (f[iw] == 0) >= 2))
+        j->kind = (j->p[3] == 0) ? kTriClique : kTriTailedClique;
+      else
+        j->kind = kTriGeneral;
+      break;
+    }
+    default: {  // FC:666-692
+      int anchors = 0;
+      for (int i = 0; i < mo.anchorsets; i++) anchors |= mo.anch[i];
+      const int nonanch = anchors ^ ((1 << mo.nodes) - 1);
+      const int top = ((1 << mo.nodes) - 1) ^ nonanch;
+      j->p[0] = mo.nodes;
+      j->p[1] = mo.startdeg;
+      for (int i = 0; i < 5; i++) { j->p[2 + i] = mo.destdeg[i]; j->p[7 + i] = mo.list[i]; }
+      j->p[12] = nonanch;
+      j->p[13] = top;
+      for (int i = 0; i < mo.anchorsets; i++) j->fcnt_by_mask[mo.anch[i]] = mo.fcnt[i];
+      // evaluation plan: anchor sets last to first, each over its superset regions
+      int L = 0;
+      for (int asi = mo.anchorsets - 1; asi >= 0; asi--) {
+        const int S = mo.anch[asi];
+        int R = S;
+        for (;;) {
+          if (L >= kMaxPlanLevels) { if (err) *err = "motif too complex: fringe plan exceeds 96 levels"; return -7; }
+          j->lev_set[L] = (int8_t)S;
+          j->lev_reg[L] = (int8_t)R;
+          j->lev_last[L] = (R == top);
+          j->lev_final[L] = (R == top) && (asi == 0);
+          L++;
+          if (R == top) break;
+          R = (((R | nonanch) + 1) | S) & ~nonanch;
+        }
+      }
+      j->nlev = L;
+      break;
+    }
+  }
+  return 0;
+}
+
+// FC:121-186 (matchingOrder2ECLgraph): core vertices in matching order, then the
+// fringes of each anchor set in order, sorted lists.
+HostGraph motif_graph(const MatchingOrder& mo)
+{
+  int n = mo.nodes;
+  for (int i = 0; i < mo.anchorsets; i++) n += mo.fcnt[i];
+  std::vector<std::vector<int>> nb(n);
+  for (int i = 0; i < mo.nodes - 1; i++)
+    for (int k = 0; k <= i; k++)
+      if (mo.list[i] & (1 << k)) { nb[i + 1].push_back(k); nb[k].push_back(i + 1); }
+  int v = mo.nodes;
+  for (int i = 0; i < mo.anchorsets; i++)
+    for (int k = 0; k < mo.fcnt[i]; k++, v++)
+      for (int c = 0; c < kMaxCore; c++)
+        if (mo.anch[i] & (1 << c)) { nb[c].push_back(v); nb[v].push_back(c); }
+  HostGraph g;
+  g.n = n;
+  g.nindex.assign(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    std::sort(nb[i].begin(), nb[i].end());
+    g.nlist.insert(g.nlist.end(), nb[i].begin(), nb[i].end());
+    g.nindex[i + 1] = (int32_t)g.nlist.size();
+  }
+  g.m = (int64_t)g.nlist.size();
+  return g;
+}
+
+}  // namespace fsgc

@@ -1,0 +1,232 @@
+"""ctypes binding of libfsgc.so (the C ABI in include/fsgc.h).
+
+This is the host-side mirror used by the tests and the benchmark: graphs are made
+resident with `Graph.*`, motifs are 139-byte matching orders (`bytes`), and counts
+come back as `Result` with the exact unsigned 128-bit value and the u64 value the
+reference's fringeCount prints. There is no fallback: if libfsgc.so is missing the
+import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libfsgc.so")
+MO_BYTES = 139
+
+FLAG_OVF64 = 1 << 0
+FLAG_OVF128 = 1 << 1
+FLAG_REF_NCK_INEXACT = 1 << 2
+FLAG_AUT_REMAINDER = 1 << 3
+FLAG_REF_AUT_TRUNC = 1 << 4
+
+CLASS_NAMES = ["vertex", "edge", "wedge", "triangle", "general"]
+
+
+class _Result(C.Structure):
+    _fields_ = [
+        ("exact_lo", C.c_uint64), ("exact_hi", C.c_uint64), ("ref_compat", C.c_uint64),
+        ("sg_lo", C.c_uint64), ("sg_hi", C.c_uint64), ("aut", C.c_uint64),
+        ("core_matches", C.c_uint64), ("flags", C.c_uint32), ("motif_class", C.c_int32),
+        ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
+    ]
+
+
+class Tune(C.Structure):
+    _fields_ = [
+        ("block", C.c_int32), ("grid", C.c_int32), ("chunk", C.c_int32), ("hub_degree", C.c_int32),
+        ("lds_cap", C.c_int32), ("slice_mod", C.c_int32), ("slice_rem", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libfsgc.so not built at {LIB_PATH}; run `make -C fringe-sgc_amd`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "fsgc_init": (C.c_int, [C.c_int]),
+        "fsgc_graph_load_egr": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+        "fsgc_graph_from_csr": (C.c_int, [C.c_int32, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_void_p)]),
+        "fsgc_graph_validate": (C.c_int, [C.c_void_p]),
+        "fsgc_graph_info": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+        "fsgc_graph_free": (None, [C.c_void_p]),
+        "fsgc_count": (C.c_int, [C.c_void_p, C.c_char_p, P(Tune), P(_Result)]),
+        "fsgc_count_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P(Tune), P(_Result), P(C.c_double)]),
+        "fsgc_preprocess": (C.c_int, [C.c_int, C.c_int, P(C.c_int), C.c_char_p, C.c_char_p, C.c_size_t,
+                                      C.c_char_p, C.c_size_t]),
+        "fsgc_preprocess_pattern": (C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),
+        "fsgc_print_matching_order": (C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+        "fsgc_generate": (C.c_int, [C.c_char_p, P(C.c_int32), P(C.c_int32), P(P(C.c_int32)), P(P(C.c_int32))]),
+        "fsgc_graph_generate": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+        "fsgc_free_host": (None, [C.c_void_p]),
+        "fsgc_random_motifs": (C.c_int, [C.c_uint64, C.c_int, C.c_char_p]),
+        "fsgc_last_error": (C.c_char_p, []),
+        "fsgc_strerror": (C.c_char_p, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class FsgcError(RuntimeError):
+    pass
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise FsgcError(f"{lib.fsgc_strerror(rc).decode()} ({rc}): {lib.fsgc_last_error().decode()}")
+
+
+@dataclass
+class Result:
+    exact: int
+    ref_compat: int
+    sg: int
+    aut: int
+    core_matches: int
+    flags: int
+    motif_class: int
+    kernel_ms: float
+    wall_ms: float
+
+    @property
+    def class_name(self) -> str:
+        return CLASS_NAMES[self.motif_class]
+
+
+def _result(r: _Result) -> Result:
+    return Result(exact=r.exact_lo | (r.exact_hi << 64), ref_compat=r.ref_compat, sg=r.sg_lo | (r.sg_hi << 64),
+                  aut=r.aut, core_matches=r.core_matches, flags=r.flags, motif_class=r.motif_class,
+                  kernel_ms=r.kernel_ms, wall_ms=r.wall_ms)
+
+
+def init(device: int = 0):
+    _check(lib.fsgc_init(device))
+
+
+def _csr_ptrs(nindex, nlist):
+    import numpy as np
+    ni = np.ascontiguousarray(nindex, dtype=np.int32)
+    nl = np.ascontiguousarray(nlist, dtype=np.int32)
+    return ni, nl, ni.ctypes.data_as(C.POINTER(C.c_int32)), nl.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Graph:
+    """A CSR graph resident in GPU memory."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_csr(cls, nindex, nlist) -> "Graph":
+        ni, nl, pi, pl = _csr_ptrs(nindex, nlist)
+        h = C.c_void_p()
+        _check(lib.fsgc_graph_from_csr(len(ni) - 1, len(nl), pi, pl, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_egr(cls, path: str) -> "Graph":
+        h = C.c_void_p()
+        _check(lib.fsgc_graph_load_egr(path.encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, spec: str) -> "Graph":
+        h = C.c_void_p()
+        _check(lib.fsgc_graph_generate(spec.encode(), C.byref(h)))
+        return cls(h.value)
+
+    def info(self):
+        n, m, d = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib.fsgc_graph_info(self._h, C.byref(n), C.byref(m), C.byref(d)))
+        return n.value, m.value, d.value
+
+    def validate(self):
+        _check(lib.fsgc_graph_validate(self._h))
+
+    def count(self, mo: bytes, tune: Tune | None = None) -> Result:
+        assert len(mo) == MO_BYTES
+        r = _Result()
+        _check(lib.fsgc_count(self._h, mo, C.byref(tune) if tune else None, C.byref(r)))
+        return _result(r)
+
+    def count_batch(self, mos: list[bytes], tune: Tune | None = None):
+        k = len(mos)
+        buf = b"".join(mos)
+        out = (_Result * max(1, k))()
+        ms = C.c_double()
+        _check(lib.fsgc_count_batch(self._h, buf, k, C.byref(tune) if tune else None, out, C.byref(ms)))
+        return [_result(out[i]) for i in range(k)], ms.value
+
+    def free(self):
+        if self._h:
+            lib.fsgc_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def generate_csr(spec: str):
+    """Host CSR arrays (numpy int32) for a synthetic graph spec."""
+    import numpy as np
+    n, m = C.c_int32(), C.c_int32()
+    pi, pl = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+    _check(lib.fsgc_generate(spec.encode(), C.byref(n), C.byref(m), C.byref(pi), C.byref(pl)))
+    try:
+        ni = np.ctypeslib.as_array(pi, shape=(n.value + 1,)).copy()
+        nl = np.ctypeslib.as_array(pl, shape=(max(1, m.value),)).copy()[: m.value]
+    finally:
+        lib.fsgc_free_host(pi)
+        lib.fsgc_free_host(pl)
+    return ni, nl
+
+
+def preprocess(nodes: int, edges) -> tuple[bytes, str]:
+    flat = [int(x) for e in edges for x in e]
+    arr = (C.c_int * max(1, len(flat)))(*flat)
+    mo = C.create_string_buffer(MO_BYTES)
+    log = C.create_string_buffer(1 << 16)
+    _check(lib.fsgc_preprocess(nodes, len(flat) // 2, arr, mo, log, len(log), None, 0))
+    return mo.raw, log.value.decode()
+
+
+def preprocess_pattern(number: int) -> tuple[bytes, str]:
+    mo = C.create_string_buffer(MO_BYTES)
+    log = C.create_string_buffer(1 << 16)
+    _check(lib.fsgc_preprocess_pattern(number, mo, log, len(log), None, 0))
+    return mo.raw, log.value.decode()
+
+
+def random_motifs(seed: int, count: int) -> list[bytes]:
+    buf = C.create_string_buffer(MO_BYTES * max(1, count))
+    _check(lib.fsgc_random_motifs(seed, count, buf))
+    return [buf.raw[i * MO_BYTES:(i + 1) * MO_BYTES] for i in range(count)]
+
+
+def print_matching_order(mo: bytes) -> str:
+    out = C.create_string_buffer(8192)
+    _check(lib.fsgc_print_matching_order(mo, out, len(out)))
+    return out.value.decode()
+
+
+def live_fields(mo: bytes) -> dict:
+    """The fields of a matching order that carry meaning (the reference leaves the
+    rest uninitialised)."""
+    nodes, nas = mo[0], mo[1]
+    return {
+        "nodes": nodes, "anchorsets": nas, "startdeg": mo[2],
+        "destdeg": list(mo[3:3 + max(0, nodes - 1)]), "list": list(mo[8:8 + max(0, nodes - 1)]),
+        "anch": list(mo[13:13 + nas]), "fcnt": list(mo[76:76 + nas]),
+    }

@@ -1,0 +1,122 @@
+/*
+ * fsgc.h — C ABI of the MI355X Fringe-SGC counting library (libfsgc.so).
+ *
+ * The library counts non-induced occurrences of a motif (given as the reference's
+ * 139-byte matching order, "motif.mo") in an undirected graph resident in GPU memory
+ * (given as the reference's ECLgraph CSR). Plain pointers and sizes only; every
+ * function returns 0 on success or a negative errno-style code (see fsgc_strerror);
+ * nothing here calls exit().
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   fsgc_graph_load_egr  readECLgraph + host-to-device copy   src/ECLgraph.h:58-85, src/fringeCount.cu:2285-2297
+ *   fsgc_graph_from_csr  the same from caller memory          src/fringeCount.cu:2292-2297
+ *   fsgc_graph_free      freeECLgraph + cudaFree              src/ECLgraph.h:88-96, src/fringeCount.cu:2325-2329
+ *   fsgc_count           occurrences(mo, d_g, wl, ...)         src/fringeCount.cu:2199-2262
+ *   fsgc_count_batch     (no counterpart: the reference counts one motif per process)
+ *   fsgc_preprocess      fringePreprocess n s d s d ...        src/fringePreprocess.cpp:287-365
+ *   fsgc_preprocess_pattern  fringePreprocess <pattern#>       src/fringePreprocess.cpp:296-298, src/patterns.h:57-354
+ *   fsgc_print_matching_order  printMatchingOrder              src/fringes.h:61-98
+ */
+#ifndef FSGC_H
+#define FSGC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSGC_MO_BYTES 139
+
+/* result flags */
+#define FSGC_OVF64 (1u << 0)           /* exact count >= 2^64: the u64 the reference prints wrapped */
+#define FSGC_OVF128 (1u << 1)          /* an exact intermediate reached 2^128: count invalid */
+#define FSGC_REF_NCK_INEXACT (1u << 2) /* the reference's u64 nCk (fringeCount.cu:195) would have wrapped */
+#define FSGC_AUT_REMAINDER (1u << 3)   /* class sum not divisible by the automorphism sum */
+#define FSGC_REF_AUT_TRUNC (1u << 4)   /* the reference's int automorphism count (fringeCount.cu:2213) truncates */
+
+/* motif classes (core size 1, 2, 3 open, 3 closed, 4-6) */
+#define FSGC_CLASS_VERTEX 0
+#define FSGC_CLASS_EDGE 1
+#define FSGC_CLASS_WEDGE 2
+#define FSGC_CLASS_TRIANGLE 3
+#define FSGC_CLASS_GENERAL 4
+
+typedef struct fsgc_graph fsgc_graph;
+
+typedef struct fsgc_result {
+  uint64_t exact_lo, exact_hi; /* exact count = floor(S_G / aut), unsigned 128-bit */
+  uint64_t ref_compat;         /* the u64 the reference computes: (S_G mod 2^64) / aut */
+  uint64_t sg_lo, sg_hi;       /* S_G: the class sum over the data graph, before / aut */
+  uint64_t aut;                /* automorphism sum (the reference prints it as "automorphisms") */
+  uint64_t core_matches;       /* core embeddings for which a fringe formula was evaluated */
+  uint32_t flags;              /* FSGC_* flags */
+  int32_t motif_class;         /* FSGC_CLASS_* */
+  double kernel_ms;            /* device time of the counting launch on the data graph */
+  double wall_ms;              /* host wall time of the whole call */
+} fsgc_result;
+
+typedef struct fsgc_tune {
+  int32_t block;       /* threads per workgroup; 0 = default (256) */
+  int32_t grid;        /* persistent workgroups; 0 = 8 per CU */
+  int32_t chunk;       /* roots per work item; 0 = default */
+  int32_t hub_degree;  /* roots above this degree get items of their own; 0 = default */
+  int32_t lds_cap;     /* longest neighbour list staged in LDS; 0 = default */
+  int32_t slice_mod;   /* > 1: count only roots u with u % slice_mod == slice_rem */
+  int32_t slice_rem;
+  int32_t reserved;
+} fsgc_tune;
+
+/* Selects the GPU (the reference hard-codes device 0, fringeCount.cu:2276). */
+int fsgc_init(int device);
+
+/* Graph residency. The graph must be undirected with sorted, duplicate-free lists and
+   no self loops (fringeCount.cu:50-56); fsgc_graph_validate checks that. */
+int fsgc_graph_load_egr(const char* path, fsgc_graph** out);
+int fsgc_graph_from_csr(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist, fsgc_graph** out);
+int fsgc_graph_validate(const fsgc_graph* g);
+int fsgc_graph_info(const fsgc_graph* g, int32_t* nodes, int32_t* edges, int32_t* maxdeg);
+void fsgc_graph_free(fsgc_graph* g);
+
+/* Counts one motif (one persistent launch on the data graph, plus one on the motif
+   graph for the automorphism sum, as the reference does). tune may be NULL. */
+int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* tune, fsgc_result* out);
+
+/* Counts k motifs in one persistent launch over one graph; out[i] is identical to
+   fsgc_count of mos + i*139 alone. batch_ms receives the device time of the launch. */
+int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* tune, fsgc_result* out,
+                     double* batch_ms);
+
+/* Motif compiler: pattern with `nodes` vertices and `nedges` edges (pairs in `edges`)
+   -> 139-byte matching order (unused slots zero). `log` receives the reference's
+   report text (may be NULL), `dot` the motif.dot text (may be NULL). */
+int fsgc_preprocess(int nodes, int nedges, const int* edges, uint8_t mo[FSGC_MO_BYTES], char* log, size_t logcap,
+                    char* dot, size_t dotcap);
+int fsgc_preprocess_pattern(int number, uint8_t mo[FSGC_MO_BYTES], char* log, size_t logcap, char* dot,
+                            size_t dotcap);
+int fsgc_print_matching_order(const uint8_t mo[FSGC_MO_BYTES], char* out, size_t cap);
+
+/* Synthetic inputs (the reference downloads its graphs, setup.sh:3-7; these replace
+   that offline). spec is one of
+     "er:<n>:<m_undirected>:<seed>"
+     "rmat:<scale>:<edge_factor>:<a>:<b>:<c>:<seed>"
+     "chunglu:<n>:<gamma>:<mean_degree>:<max_degree>:<seed>"
+     "egr:<path>"
+   fsgc_generate returns host arrays to be released with fsgc_free_host. */
+int fsgc_generate(const char* spec, int32_t* nodes, int32_t* edges, int32_t** nindex, int32_t** nlist);
+int fsgc_graph_generate(const char* spec, fsgc_graph** out);
+void fsgc_free_host(void* p);
+/* count random motifs (core 2..5 vertices, density U[0.4,1), 0..6 degree-1 fringes)
+   from `seed`, preprocessed into count * 139 bytes of matching orders. */
+int fsgc_random_motifs(uint64_t seed, int count, uint8_t* mos);
+
+/* Text of the last error on this thread, and of an error code. */
+const char* fsgc_last_error(void);
+const char* fsgc_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSGC_H */
