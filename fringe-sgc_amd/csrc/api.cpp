@@ -419,6 +419,24 @@ int fsgc_random_motifs(uint64_t seed, int count, uint8_t* mos)
   return 0;
 }
 
+int fsgc_random_patterns(uint64_t seed, int count, int* nodes, int* nedges, int* edges)
+{
+  if (!nodes || !nedges || !edges || count < 0) return fail(-22, "null argument");
+  std::vector<fsgc::Pattern> ps;
+  fsgc::gen_random_motifs(seed, count, 2, 5, 6, &ps);
+  for (int i = 0; i < count; i++) {
+    nodes[i] = ps[i].nodes;
+    int k = 0;
+    for (int a = 0; a < ps[i].nodes; a++)
+      for (uint32_t m = ps[i].adj[a]; m; m &= m - 1) {
+        const int b = __builtin_ctz(m);
+        if (a < b && k < 64) { edges[i * 128 + 2 * k] = a; edges[i * 128 + 2 * k + 1] = b; k++; }
+      }
+    nedges[i] = k;
+  }
+  return 0;
+}
+
 const char* fsgc_last_error(void) { return t_err.c_str(); }
 
 const char* fsgc_strerror(int code)

@@ -63,6 +63,7 @@ def _load():
         "fsgc_graph_generate": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
         "fsgc_free_host": (None, [C.c_void_p]),
         "fsgc_random_motifs": (C.c_int, [C.c_uint64, C.c_int, C.c_char_p]),
+        "fsgc_random_patterns": (C.c_int, [C.c_uint64, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
         "fsgc_last_error": (C.c_char_p, []),
         "fsgc_strerror": (C.c_char_p, [C.c_int]),
     }
@@ -213,6 +214,16 @@ def random_motifs(seed: int, count: int) -> list[bytes]:
     buf = C.create_string_buffer(MO_BYTES * max(1, count))
     _check(lib.fsgc_random_motifs(seed, count, buf))
     return [buf.raw[i * MO_BYTES:(i + 1) * MO_BYTES] for i in range(count)]
+
+
+def random_patterns(seed: int, count: int):
+    """The random motifs as (nodes, edge list) before preprocessing."""
+    nodes = (C.c_int * max(1, count))()
+    ned = (C.c_int * max(1, count))()
+    edges = (C.c_int * (128 * max(1, count)))()
+    _check(lib.fsgc_random_patterns(seed, count, nodes, ned, edges))
+    return [(nodes[i], [(edges[i * 128 + 2 * k], edges[i * 128 + 2 * k + 1]) for k in range(ned[i])])
+            for i in range(count)]
 
 
 def print_matching_order(mo: bytes) -> str:

@@ -110,6 +110,9 @@ void fsgc_free_host(void* p);
 /* count random motifs (core 2..5 vertices, density U[0.4,1), 0..6 degree-1 fringes)
    from `seed`, preprocessed into count * 139 bytes of matching orders. */
 int fsgc_random_motifs(uint64_t seed, int count, uint8_t* mos);
+/* the same motifs before preprocessing: nodes[i], nedges[i] and up to 64 edge pairs
+   at edges + i * 128 */
+int fsgc_random_patterns(uint64_t seed, int count, int* nodes, int* nedges, int* edges);
 
 /* Text of the last error on this thread, and of an error code. */
 const char* fsgc_last_error(void);
