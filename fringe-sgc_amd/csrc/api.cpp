@@ -147,6 +147,7 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   std::vector<const fsgc::DeviceGraph*> graphs;
   std::vector<fsgc::DeviceGraph> mgs(ps.size());
   std::vector<int> data_job(ps.size(), -1), aut_job(ps.size(), -1);
+  const bool pivot = !(tune && tune->schedule == 1);
   int rc = 0;
   for (size_t i = 0; i < ps.size(); i++) {
     Prepared& p = ps[i];
@@ -161,19 +162,38 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
       jobs.push_back(hj);
       graphs.push_back(&mgs[i]);
     }
-    if (!p.trivial) {
+    if (!p.trivial && !(pivot && p.job.cls == fsgc::kWedge)) {
       data_job[i] = (int)jobs.size();
       jobs.push_back(p.job);
       graphs.push_back(&g->dg);
     }
   }
   std::vector<fsgc::JobResult> jr;
-  if (!rc) rc = fsgc::device_run_jobs(jobs, graphs, to_tune(tune), nullptr, &jr, kernel_ms, &err);
+  double ms_total = 0, ms = 0;
+  if (!rc) rc = fsgc::device_run_jobs(jobs, graphs, to_tune(tune), nullptr, &jr, &ms, &err);
+  ms_total += ms;
+  // wedge-class motifs on the data graph: pivot schedule, up to kMaxPivotJobs per launch
+  std::vector<fsgc::JobResult> wres(ps.size());
+  if (!rc && pivot) {
+    std::vector<size_t> wl;
+    for (size_t i = 0; i < ps.size(); i++)
+      if (!ps[i].trivial && ps[i].job.cls == fsgc::kWedge) wl.push_back(i);
+    for (size_t b = 0; b < wl.size() && !rc; b += fsgc::kMaxPivotJobs) {
+      std::vector<fsgc::JobDesc> wj;
+      for (size_t k = b; k < std::min(wl.size(), b + fsgc::kMaxPivotJobs); k++) wj.push_back(ps[wl[k]].job);
+      std::vector<fsgc::JobResult> r;
+      rc = fsgc::device_run_wedge_pivot(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      ms_total += ms;
+      for (size_t k = 0; k < r.size(); k++) wres[wl[b + k]] = r[k];
+    }
+  }
+  if (kernel_ms) *kernel_ms = ms_total;
   for (auto& mg : mgs) fsgc::device_graph_free(&mg);
   if (rc) return fail(rc, err);
   res->assign(ps.size(), fsgc::JobResult());
   for (size_t i = 0; i < ps.size(); i++) {
     if (data_job[i] >= 0) (*res)[i] = jr[data_job[i]];
+    else if (pivot && !ps[i].trivial && ps[i].job.cls == fsgc::kWedge) (*res)[i] = wres[i];
     if (aut_job[i] >= 0) {
       Prepared& p = ps[i];
       p.aut.sum = jr[aut_job[i]].sum;

@@ -33,6 +33,14 @@ struct DeviceGraph {
   int32_t* idx = nullptr;   // n + 1
   int32_t* adj = nullptr;   // m
   std::vector<int32_t> h_idx;  // host copy of nindex (used to cut work items)
+  // derived once per graph for the pivot schedule
+  bool derived_ready = false;
+  int32_t* tsup = nullptr;      // m: |N(u) & N(x)| per directed edge
+  long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
+  int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
+  long long tl_total = 0;
+  long long* piv_cost = nullptr;         // n: sum of neighbour degrees
+  std::vector<long long> h_piv_cost;
 };
 
 struct JobResult {
@@ -40,6 +48,13 @@ struct JobResult {
   uint64_t matches = 0;
   uint32_t flags = 0;
 };
+
+// pivot.hip
+int device_derived_build(DeviceGraph* g, void* stream, double* ms, std::string* err);
+void device_derived_free(DeviceGraph* g);
+constexpr int kMaxPivotJobs = 8;
+int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                           std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
 // kernels.hip
 int device_init(int device, std::string* err);

@@ -14,273 +14,11 @@
 #include <cstdio>
 #include <cstring>
 
+#include "devmath.h"
 #include "fsgc_plan.h"
 
 namespace fsgc {
 namespace dev {
-
-typedef unsigned __int128 u128;
-typedef unsigned long long ull;
-
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kVennSlots = 16;  // general-core Venn vectors staged per wave
-
-enum { iu = 1, iv = 2, iw = 4, iuv = 3, iuw = 5, ivw = 6, iuvw = 7 };
-
-// inverses of the odd numbers 1, 3, ..., 63 modulo 2^128 (for exact division)
-__constant__ ull c_inv_lo[32];
-__constant__ ull c_inv_hi[32];
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-template <typename T>
-__device__ __forceinline__ T bcast(T x, int src) { return __shfl(x, src, kWave); }
-
-__device__ __forceinline__ int wave_sum(int x)
-{
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-  return x;
-}
-
-// ------------------------------------------------------------------ exact arithmetic
-
-__device__ __forceinline__ u128 mul_lo128(u128 a, u128 b)
-{
-  const ull al = (ull)a, ah = (ull)(a >> 64), bl = (ull)b, bh = (ull)(b >> 64);
-  u128 p = (u128)al * bl;
-  const ull cross = ah * bl + al * bh;
-  return p + ((u128)cross << 64);
-}
-
-// a * b with overflow detection (flag kOvf128)
-__device__ __forceinline__ u128 mulx(u128 a, u128 b, uint32_t& fl)
-{
-  const ull al = (ull)a, ah = (ull)(a >> 64), bl = (ull)b, bh = (ull)(b >> 64);
-  const u128 p = (u128)al * bl;
-  if ((ah | bh) == 0) return p;
-  if (ah && bh) fl |= kOvf128;
-  const u128 c = (u128)ah * bl + (u128)al * bh;
-  if ((ull)(c >> 64)) fl |= kOvf128;
-  const u128 r = p + ((u128)(ull)c << 64);
-  if (r < p) fl |= kOvf128;
-  return r;
-}
-
-__device__ __forceinline__ u128 addx(u128 a, u128 b, uint32_t& fl)
-{
-  const u128 r = a + b;
-  if (r < a) fl |= kOvf128;
-  return r;
-}
-
-// x / i for 1 <= i <= 63 when the division is exact: shift out the power of two, then
-// multiply by the inverse of the odd part modulo 2^128.
-__device__ __forceinline__ u128 div_exact(u128 x, int i)
-{
-  const int s = __builtin_ctz(i);
-  const int o = i >> s;
-  x >>= s;
-  if (o == 1) return x;
-  const u128 inv = ((u128)c_inv_hi[o >> 1] << 64) | c_inv_lo[o >> 1];
-  return mul_lo128(x, inv);
-}
-
-// FC:189-197 semantics: k = min(k, n - k); k < 0 -> 0; k == 0 -> 1; otherwise the exact
-// binomial. The reference's u64 loop wraps iff k * C(n, k) >= 2^64 (kRefNckInexact).
-__device__ u128 ncr(int n, int k, uint32_t& fl)
-{
-  k = min(k, n - k);
-  if (k < 0) return 0;
-  if (k == 0) return 1;
-  if (k == 1) return (u128)(unsigned)n;
-  // fast path: all intermediates i * C(n-k+i, i) <= k * C(n, k) fit 64 bits
-  ull r = 1;
-  int i = 1;
-  for (; i <= k; i++) {
-    const ull f = (ull)(unsigned)(n - k + i);
-    if (__umul64hi(r, f) != 0) break;
-    r = (r * f) / (ull)i;
-  }
-  u128 R = r;
-  if (i <= k) {
-    fl |= kRefNckInexact;
-    for (; i <= k; i++) {
-      R = mulx(R, (u128)(unsigned)(n - k + i), fl);
-      R = div_exact(R, i);
-    }
-    return R;
-  }
-  if (__umul64hi(r, (ull)k) != 0) fl |= kRefNckInexact;
-  return R;
-}
-
-// ------------------------------------------------------------------ graph helpers
-
-struct G {
-  const int* __restrict__ idx;
-  const int* __restrict__ adj;
-  __device__ __forceinline__ int beg(int u) const { return idx[u]; }
-  __device__ __forceinline__ int end(int u) const { return idx[u + 1]; }
-  __device__ __forceinline__ int deg(int u) const { return idx[u + 1] - idx[u]; }
-};
-
-// membership of x in the sorted range adj[l, r)
-__device__ __forceinline__ bool contains(const int* __restrict__ a, int l, int r, int x)
-{
-  int lo = l, hi = r;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo < r && a[lo] == x;
-}
-
-// |N(a) & N(b)|, lane-strided over the shorter list (wave-cooperative, uniform args)
-__device__ int coop_common(const G& g, int ab, int ae, int bb, int be)
-{
-  if (ae - ab > be - bb) { int t = ab; ab = bb; bb = t; t = ae; ae = be; be = t; }
-  int c = 0;
-  for (int i = ab + lane_id(); i < ae; i += kWave) c += contains(g.adj, bb, be, g.adj[i]);
-  return wave_sum(c);
-}
-
-// owner of CSR position j, searching vertices [lo, hi]
-__device__ __forceinline__ int owner_of(const int* __restrict__ idx, int lo, int hi, long long j)
-{
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (idx[mid] <= j) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-// ------------------------------------------------------------------ accumulation
-
-struct LaneAcc {
-  ull lo = 0, hi = 0, matches = 0;
-  uint32_t fl = 0;
-  __device__ __forceinline__ void add(u128 x)
-  {
-    const u128 s = (((u128)hi << 64) | lo);
-    const u128 r = s + x;
-    if (r < s) fl |= kOvf128;
-    lo = (ull)r;
-    hi = (ull)(r >> 64);
-  }
-};
-
-__device__ void flush(LaneAcc& a, Accum* out)
-{
-  ull lo = a.lo, hi = a.hi, mt = a.matches;
-  uint32_t fl = a.fl;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const ull olo = __shfl_xor(lo, o, kWave), ohi = __shfl_xor(hi, o, kWave);
-    const ull nlo = lo + olo;
-    const ull c = nlo < lo;
-    const u128 nh = (u128)hi + ohi + c;
-    if ((ull)(nh >> 64)) fl |= kOvf128;
-    lo = nlo;
-    hi = (ull)nh;
-    mt += __shfl_xor(mt, o, kWave);
-    fl |= __shfl_xor(fl, o, kWave);
-  }
-  if (lane_id() == 0 && (lo | hi | mt | fl)) {
-    const ull old = atomicAdd((ull*)&out->lo, lo);
-    const ull carry = (old + lo) < old;
-    const ull add_hi = hi + carry;
-    if (add_hi < hi) fl |= kOvf128;
-    if (add_hi) {
-      const ull oh = atomicAdd((ull*)&out->hi, add_hi);
-      if (oh + add_hi < oh) fl |= kOvf128;
-    }
-    if (mt) atomicAdd((ull*)&out->matches, mt);
-    if (fl) atomicOr(&out->flags, fl);
-  }
-  a = LaneAcc();
-}
-
-// ------------------------------------------------------------------ fringe formulas
-
-// FC:233-327 (fc3): ways to place the fringes of the 7 anchor sets of a 3-vertex core
-// disjointly into the 7 Venn regions (loop order uvw, uv, uw, vw, u; v and w merged).
-__device__ u128 fc3(const int* fr, const int* venn, uint32_t& fl)
-{
-  const int fr_uvw = fr[iuvw], fr_uv = fr[iuv], fr_uw = fr[iuw], fr_vw = fr[ivw];
-  const int fr_u = fr[iu], fr_v = fr[iv], fr_w = fr[iw];
-  int vd_uvw = venn[iuvw], vd_uv = venn[iuv], vd_uw = venn[iuw], vd_vw = venn[ivw];
-  const int vd_u = venn[iu], vd_v = venn[iv], vd_w = venn[iw];
-  if (vd_uvw < fr_uvw) return 0;
-  u128 count = 0;
-  const u128 cuvw = ncr(vd_uvw, fr_uvw, fl);
-  vd_uvw -= fr_uvw;
-  for (int i = min(vd_uvw, fr_uv); i >= 0; i--) {
-    if (vd_uv < fr_uv - i) break;
-    const u128 c1 = mulx(mulx(cuvw, ncr(vd_uvw, i, fl), fl), ncr(vd_uv, fr_uv - i, fl), fl);
-    vd_uvw -= i; vd_uv -= fr_uv - i;
-    for (int j = min(vd_uvw, fr_uw); j >= 0; j--) {
-      if (vd_uw < fr_uw - j) break;
-      const u128 c2 = mulx(mulx(c1, ncr(vd_uvw, j, fl), fl), ncr(vd_uw, fr_uw - j, fl), fl);
-      vd_uvw -= j; vd_uw -= fr_uw - j;
-      for (int k = min(vd_uvw, fr_vw); k >= 0; k--) {
-        if (vd_vw < fr_vw - k) break;
-        const u128 c3 = mulx(mulx(c2, ncr(vd_uvw, k, fl), fl), ncr(vd_vw, fr_vw - k, fl), fl);
-        vd_uvw -= k; vd_vw -= fr_vw - k;
-        for (int a = min(vd_uvw, fr_u); a >= 0; a--) {
-          const u128 ta = mulx(c3, ncr(vd_uvw, a, fl), fl);
-          vd_uvw -= a;
-          int vd_uvw_vw = vd_uvw + vd_vw;
-          for (int b = min(vd_uv, fr_u - a); b >= 0; b--) {
-            const u128 tb = mulx(ta, ncr(vd_uv, b, fl), fl);
-            vd_uv -= b;
-            for (int c = min(vd_uw, fr_u - a - b); c >= 0; c--) {
-              if (vd_u < fr_u - a - b - c) break;
-              const u128 tc = mulx(mulx(tb, ncr(vd_uw, c, fl), fl), ncr(vd_u, fr_u - a - b - c, fl), fl);
-              vd_uw -= c;
-              for (int d = min(vd_uvw_vw, fr_v); d >= 0; d--) {
-                if (vd_uv + vd_v < fr_v - d) break;
-                const u128 td = mulx(mulx(tc, ncr(vd_uvw_vw, d, fl), fl), ncr(vd_uv + vd_v, fr_v - d, fl), fl);
-                vd_uvw_vw -= d;
-                if (vd_uvw_vw + vd_uw + vd_w >= fr_w)
-                  count = addx(count, mulx(td, ncr(vd_uvw_vw + vd_uw + vd_w, fr_w, fl), fl), fl);
-                vd_uvw_vw += d;
-              }
-              vd_uw += c;
-            }
-            vd_uv += b;
-          }
-          vd_uvw += a;
-        }
-        vd_uvw += k; vd_vw += fr_vw - k;
-      }
-      vd_uvw += j; vd_uw += fr_uw - j;
-    }
-    vd_uvw += i; vd_uv += fr_uv - i;
-  }
-  return count;
-}
-
-// FC:2030-2048 (general_edge_core) and FC:1381-1406 (wedge): the two-vertex closed
-// form; `corr` is how many core vertices each list holds besides the shared ones
-// (1 for an edge core, 1 or 2 for wedge endpoints).
-__device__ u128 pair_formula(int da, int db, int c_ab, int corr, int f_a, int f_ab, int f_b, uint32_t& fl)
-{
-  const int a_ab = c_ab - f_ab;
-  if (a_ab < 0) return 0;
-  const int ca = da - (corr + c_ab), cb = db - (corr + c_ab);
-  const int bound = min(f_a, a_ab);
-  u128 acc = 0;
-  for (int j = 0; j <= bound; j++) {
-    const int k = f_a - j;
-    u128 term = mulx(ncr(ca, k, fl), ncr(a_ab + cb - j, f_b, fl), fl);
-    if (f_a != f_b) term = addx(term, mulx(ncr(cb, k, fl), ncr(a_ab + ca - j, f_b, fl), fl), fl);
-    acc = addx(acc, mulx(term, ncr(a_ab, j, fl), fl), fl);
-  }
-  return mulx(acc, ncr(c_ab, f_ab, fl), fl);
-}
 
 // FC:408-486 (functionX) as a per-lane depth-first walk over a precomputed plan of
 // (anchor set, region) levels: each level places some of the set's fringes into the
@@ -758,24 +496,14 @@ __global__ __launch_bounds__(kBlock) void k_count(const JobDesc* __restrict__ jo
     }                                                                                 \
   } while (0)
 
-static bool g_inv_ready = false;
-
 int device_init(int device, std::string* err)
 {
   HIP_TRY(hipSetDevice(device));
-  if (!g_inv_ready) {
-    typedef unsigned __int128 u128;
-    unsigned long long lo[32], hi[32];
-    for (int k = 0; k < 32; k++) {
-      const u128 o = (u128)(2 * k + 1);
-      u128 x = o;  // Newton iteration for the inverse modulo 2^128
-      for (int it = 0; it < 8; it++) x = x * ((u128)2 - o * x);
-      lo[k] = (unsigned long long)x;
-      hi[k] = (unsigned long long)(x >> 64);
-    }
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(dev::c_inv_lo), lo, sizeof(lo)));
-    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(dev::c_inv_hi), hi, sizeof(hi)));
-    g_inv_ready = true;
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    if (err) *err = "no such GPU";
+    return -19;
   }
   return 0;
 }
@@ -796,6 +524,7 @@ int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err)
 
 void device_graph_free(DeviceGraph* dg)
 {
+  device_derived_free(dg);
   if (dg->idx) (void)hipFree(dg->idx);
   if (dg->adj) (void)hipFree(dg->adj);
   dg->idx = dg->adj = nullptr;

@@ -1,0 +1,544 @@
+// Wedge-core counting on the pivot schedule (gfx950).
+//
+// A wedge of the reference (FC:1513-1650, FC:1409-1477) is a centre u with two
+// neighbours v > w; its fringe formula needs d_u, d_v, d_w, the edge supports
+// t_uv = |N(u) & N(v)| and t_uw, the codegree c_vw = |N(v) & N(w)|, the triple count
+// c3 = |N(u) & N(v) & N(w)| and whether v ~ w. The reference walks N(w) per wedge
+// with two binary searches per element (sum over wedges of d_w probes). Here:
+//   * t_e for every directed edge is computed once per graph (support pass), and so
+//     are the per-edge common-neighbour lists T(u,x) as positions in N(u);
+//   * a workgroup takes a pivot w (the smaller endpoint) and scatters the 2-hop
+//     counts c(w, y) for y > w into a dense per-workgroup array D (plus an adjacency
+//     bit), so every wedge with endpoint w reads c_vw and v ~ w with one load;
+//   * for each centre u of w, c3(u, v, w) for all v is the number of x in T(u,w)
+//     with v in T(u,x): a scatter over the precomputed lists into LDS counters E
+//     indexed by position in N(u);
+//   * a wave then walks N(u) past w and evaluates every wedge job per lane.
+// The sum over wedges is the reference's, with the same per-wedge filters, formulas
+// and counted matches; only the order of evaluation differs.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "devmath.h"
+#include "fsgc_plan.h"
+
+namespace fsgc {
+namespace dev {
+
+constexpr int kPivBlock = 256;
+constexpr int kPivWaves = kPivBlock / kWave;
+constexpr int kECap = 4096;  // LDS counters per wave (positions of N(u))
+constexpr int kMaxPivotJobs = 8;
+
+struct PivotItem {
+  int32_t w;   // pivot
+  int32_t p0;  // centre positions [p0, p1) within N(w)
+  int32_t p1;
+  int32_t pad;
+};
+
+// lower_bound of x in adj[l, r)
+__device__ __forceinline__ int lower_bound_pos(const int* __restrict__ a, int l, int r, int x)
+{
+  while (l < r) {
+    const int mid = (l + r) >> 1;
+    if (a[mid] < x) l = mid + 1; else r = mid;
+  }
+  return l;
+}
+
+// ---------------------------------------------------------------- support pass
+
+// Per directed edge (u, x): |N(u) & N(x)| (count mode) or the ascending positions in
+// N(u) of the common neighbours (list mode). Light edges (shorter list <= 32) are done
+// by one lane; heavier ones cooperatively by the wave.
+template <bool kList>
+__global__ __launch_bounds__(256) void k_common(const int* __restrict__ idx, const int* __restrict__ adj, int n,
+                                                int* __restrict__ tsup, const long long* __restrict__ tl_off,
+                                                int* __restrict__ tl, int* __restrict__ head)
+{
+  const int lane = lane_id();
+  for (;;) {
+    int u = 0;
+    if (lane == 0) u = atomicAdd(head, 1);
+    u = bcast(u, 0);
+    if (u >= n) break;
+    const int ub = idx[u], ue = idx[u + 1], du = ue - ub;
+    for (int e0 = ub; e0 < ue; e0 += kWave) {
+      const int e = e0 + lane;
+      bool heavy = false;
+      int x = 0, xb = 0, xe = 0;
+      if (e < ue) {
+        x = adj[e];
+        xb = idx[x];
+        xe = idx[x + 1];
+        const int dx = xe - xb;
+        const int shorter = min(du, dx);
+        if (shorter <= 32) {
+          // serial: walk the shorter list, search the longer one
+          int c = 0;
+          long long out = kList ? tl_off[e] : 0;
+          if (du <= dx) {
+            for (int i = ub; i < ue; i++) {
+              if (contains(adj, xb, xe, adj[i])) {
+                if (kList) tl[out++] = i - ub;
+                c++;
+              }
+            }
+          } else {
+            for (int i = xb; i < xe; i++) {
+              const int y = adj[i];
+              const int p = lower_bound_pos(adj, ub, ue, y);
+              if (p < ue && adj[p] == y) {
+                if (kList) tl[out++] = p - ub;
+                c++;
+              }
+            }
+          }
+          if (!kList) tsup[e] = c;
+        } else {
+          heavy = true;
+        }
+      }
+      ull hm = __ballot(heavy);
+      while (hm) {
+        const int who = __builtin_ctzll(hm);
+        hm &= hm - 1;
+        const int X = bcast(x, who), XB = bcast(xb, who), XE = bcast(xe, who), E = bcast(e, who);
+        const int dx = XE - XB;
+        const bool walk_u = du <= dx;
+        const int sb = walk_u ? ub : XB, se = walk_u ? ue : XE;
+        int c = 0;
+        long long out = kList ? tl_off[E] : 0;
+        for (int i0 = sb; i0 < se; i0 += kWave) {
+          const int i = i0 + lane;
+          bool hit = false;
+          int pos = 0;
+          if (i < se) {
+            const int y = adj[i];
+            if (walk_u) {
+              hit = contains(adj, XB, XE, y);
+              pos = i - ub;
+            } else {
+              const int p = lower_bound_pos(adj, ub, ue, y);
+              hit = p < ue && adj[p] == y;
+              pos = p - ub;
+            }
+          }
+          const ull b = __ballot(hit);
+          if (kList && hit) tl[out + __popcll(b & ((1ull << lane) - 1))] = pos;
+          out += __popcll(b);
+          c += __popcll(b);
+        }
+        if (!kList && lane == 0) tsup[E] = c;
+        (void)X;
+      }
+    }
+  }
+}
+
+struct Widen {
+  __host__ __device__ long long operator()(const int& x) const { return (long long)x; }
+};
+
+// ---------------------------------------------------------------- pivot kernel
+
+struct WedgeEval {
+  int kind, rot, smod, srem;
+  int p[11];
+};
+
+// One wedge (centre u, endpoints v > w) for one wedge job: filters of FC:1558-1582
+// (general) or FC:1445-1457 (specialised), Venn corrections FC:1598-1602 / 1631-1636.
+template <bool kFull>
+__device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t& fl)
+{
+  if constexpr (kFull) {
+    if (!fc3_is_singleton(fr)) return fc3(fr, venn, fl);
+  }
+  return fc3_singletons(fr, venn, fl);
+}
+
+template <bool kFull>
+__device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u, int w, int du, int dv, int dw,
+                                           int tuv, int tuw, int cvw, int c3, bool tri, u128& val, uint32_t& fl)
+{
+  const int* p = J->p;
+  if (J->kind == kWedgeSpecial) {
+    if (du < 2 || dv < p[1]) return false;
+    const int thres = max(p[10], p[8] + p[9] - dv);
+    if (dw < thres) return false;
+    val = pair_formula(dv, dw, cvw - 1, tri ? 2 : 1, p[5], p[6], p[7], fl);
+    return true;
+  }
+  const int min_u = p[0], lim = p[1], sum1 = p[2], sum2 = p[3], sum3 = p[4];
+  if (du < sum1 || dv < max(sum1, sum2 - du)) return false;
+  const int thres2 = max(max(sum1, sum3 - du - dv), max(sum2 - du, sum2 - dv));
+  if (dw < thres2) return false;
+  if (tri) {
+    if (!(u < w)) return false;
+  } else {
+    const bool cond = (du >= min_u) && (dv >= lim);
+    if (!(cond && dw >= max(lim, sum2 - dv))) return false;
+  }
+  int venn[8];
+  const int c13 = tuw - c3, c23 = cvw - c3;
+  venn[0] = 0;
+  venn[iuvw] = c3;
+  venn[iuv] = tuv - c3;
+  venn[iuw] = c13;
+  venn[ivw] = c23;
+  venn[iu] = du - (tuv + c13);
+  venn[iv] = dv - (tuv + c23);
+  venn[iw] = dw - (c13 + c23 + c3);
+  u128 s = 0;
+  if (!tri) {
+    venn[iu] -= 2;
+    venn[ivw] -= 1;
+    s = fc3_any<kFull>(J->fr[0], venn, fl);
+    if (J->rot > 3) s = addx(s, fc3_any<kFull>(J->fr[3], venn, fl), fl);
+  } else {
+    venn[iuv] -= 1;
+    venn[iuw] -= 1;
+    venn[ivw] -= 1;
+    for (int q = 0; q < J->rot; q++) s = addx(s, fc3_any<kFull>(J->fr[q], venn, fl), fl);
+  }
+  val = s;
+  return true;
+}
+
+template <int NJ, bool kFull>
+__global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
+                                                           const int* __restrict__ tsup,
+                                                           const long long* __restrict__ tl_off,
+                                                           const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
+                                                           int njobs, int need_c3, const PivotItem* __restrict__ items,
+                                                           int nitems, int* __restrict__ head,
+                                                           unsigned* __restrict__ dbuf, long long dstride,
+                                                           unsigned* __restrict__ ebuf, long long estride)
+{
+  __shared__ unsigned s_e[kPivWaves][kECap];
+  __shared__ int s_item, s_next;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  unsigned* __restrict__ D = dbuf + (long long)blockIdx.x * dstride;
+  unsigned* __restrict__ Eg = ebuf + ((long long)blockIdx.x * kPivWaves + wave) * estride;
+  const G g{idx, adj};
+  LaneAcc acc[NJ];
+
+  for (;;) {
+    if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; }
+    __syncthreads();
+    const int it = s_item;
+    if (it >= nitems) break;
+    const PivotItem I = items[it];
+    const int w = I.w, wb = g.beg(w), we = g.end(w), dw = we - wb;
+
+    // Phase A: D[y] += 2 for every path w - x - y with y > w; D[x] |= 1 for x in N(w).
+    for (int i = wb + wave; i < we; i += kPivWaves) {
+      const int x = adj[i];
+      const int xb = g.beg(x), xe = g.end(x);
+      if (lane == 0 && x > w) atomicOr(&D[x], 1u);
+      const int s = lower_bound_pos(adj, xb, xe, w + 1);
+      for (int k = s + lane; k < xe; k += kWave) atomicAdd(&D[adj[k]], 2u);
+    }
+    __syncthreads();
+
+    // Phase B: centres, one per wave at a time
+    for (;;) {
+      int c = 0;
+      if (lane == 0) c = atomicAdd(&s_next, 1);
+      c = bcast(c, 0);
+      const int p = I.p0 + c;
+      if (p >= I.p1) break;
+      const int u = adj[wb + p];
+      const int ub = g.beg(u), ue = g.end(u), du = ue - ub;
+      const int tuw = tsup[wb + p];
+      const int rq = lower_bound_pos(adj, ub, ue, w) - ub;  // position of w in N(u)
+      if (rq + 1 >= du) continue;
+      bool in_slice = true;
+      if (jobs[0].smod > 1) in_slice = (u % jobs[0].smod) == jobs[0].srem;
+      if (!in_slice) continue;
+      const bool lds = du <= kECap;
+      unsigned* __restrict__ E = lds ? s_e[wave] : Eg;
+      const bool c3on = need_c3 && tuw > 0;
+      if (c3on) {
+        for (int q = rq + 1 + lane; q < du; q += kWave) E[q] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // scatter: for x in T(u, w), for y in T(u, x) with y > rq: E[y]++
+        const long long tb = tl_off[ub + rq], te = tl_off[ub + rq + 1];
+        for (long long t0 = tb; t0 < te; t0 += kWave) {
+          const long long t = t0 + lane;
+          int xq = -1;
+          if (t < te) xq = tl[t];
+          // each lane walks its own list; long lists are shared by the wave
+          long long lb = 0, le = 0;
+          if (xq >= 0) {
+            lb = tl_off[ub + xq];
+            le = tl_off[ub + xq + 1];
+            long long lo = lb, hi = le;  // first entry > rq
+            while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] <= rq) lo = mid + 1; else hi = mid; }
+            lb = lo;
+          }
+          const bool big = (le - lb) > 64;
+          if (!big) {
+            for (long long k = lb; k < le; k++) atomicAdd(&E[tl[k]], 1u);
+          }
+          ull bm = __ballot(big);
+          while (bm) {
+            const int who = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            const long long B = bcast(lb, who), Ee = bcast(le, who);
+            for (long long k = B + lane; k < Ee; k += kWave) atomicAdd(&E[tl[k]], 1u);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      // walk N(u) past w
+      for (int q0 = rq + 1; q0 < du; q0 += kWave) {
+        const int q = q0 + lane;
+        if (q < du) {
+          const int v = adj[ub + q];
+          const unsigned dv_ = D[v];
+          const bool tri = dv_ & 1u;
+          const int cvw = (int)(dv_ >> 1);
+          const int tuv = tsup[ub + q];
+          const int c3 = c3on ? (int)E[q] : 0;
+          const int dv = g.deg(v);
+#pragma unroll
+          for (int j = 0; j < NJ; j++) {
+            if (j < njobs) {
+              u128 val;
+              if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
+                acc[j].add(val);
+                acc[j].matches++;
+              }
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Phase C: clear D
+    for (int i = wb + wave; i < we; i += kPivWaves) {
+      const int x = adj[i];
+      const int xb = g.beg(x), xe = g.end(x);
+      if (lane == 0 && x > w) D[x] = 0u;
+      const int s = lower_bound_pos(adj, xb, xe, w + 1);
+      for (int k = s + lane; k < xe; k += kWave) D[adj[k]] = 0u;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; j++)
+    if (j < njobs) flush(acc[j], jobs[j].out);
+}
+
+// cost of pivot w: sum of the degrees of its neighbours
+__global__ void k_pivot_cost(const int* __restrict__ idx, const int* __restrict__ adj, int n, long long* __restrict__ cost)
+{
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n) return;
+  long long s = 0;
+  for (int i = idx[w]; i < idx[w + 1]; i++) { const int x = adj[i]; s += idx[x + 1] - idx[x]; }
+  cost[w] = s;
+}
+
+}  // namespace dev
+
+#define HIP_TRY(expr)                                                    \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) {                                              \
+      if (err) *err = std::string(#expr) + ": " + hipGetErrorString(e_); \
+      return -5;                                                         \
+    }                                                                    \
+  } while (0)
+
+void device_derived_free(DeviceGraph* g)
+{
+  if (g->tsup) (void)hipFree(g->tsup);
+  if (g->tl_off) (void)hipFree(g->tl_off);
+  if (g->tl) (void)hipFree(g->tl);
+  if (g->piv_cost) (void)hipFree(g->piv_cost);
+  g->tsup = nullptr;
+  g->tl_off = nullptr;
+  g->tl = nullptr;
+  g->piv_cost = nullptr;
+  g->derived_ready = false;
+}
+
+// Builds the per-edge supports and common-neighbour lists (once per graph).
+int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string* err)
+{
+  if (g->derived_ready) { if (ms) *ms = 0; return 0; }
+  hipStream_t stream = (hipStream_t)stream_v;
+  const int n = g->n;
+  const long long m = g->m;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
+  HIP_TRY(hipMalloc((void**)&g->tl_off, sizeof(long long) * (m + 1)));
+  HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
+  int* d_head = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_head, sizeof(int) * 2));
+  HIP_TRY(hipMemsetAsync(d_head, 0, sizeof(int) * 2, stream));
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::max(1, std::min(cus * 8, (n + 3) / 4));
+  hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
+                     (const long long*)nullptr, (int*)nullptr, d_head);
+  HIP_TRY(hipGetLastError());
+  // tl_off = exclusive prefix sum of tsup (64-bit)
+  HIP_TRY(hipMemsetAsync(g->tl_off, 0, sizeof(long long), stream));
+  size_t tmp_bytes = 0;
+  hipcub::TransformInputIterator<long long, dev::Widen, const int*> in(g->tsup, dev::Widen());
+  HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
+  void* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, tmp_bytes + 16));
+  HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
+  long long total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, g->tl_off + m, sizeof(long long), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  (void)hipFree(tmp);
+  g->tl_total = total;
+  HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
+  HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
+  hipLaunchKernelGGL(dev::k_common<true>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
+                     (const long long*)g->tl_off, g->tl, d_head + 1);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_head);
+  g->h_piv_cost.resize(n);
+  HIP_TRY(hipMemcpy(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost));
+  g->derived_ready = true;
+  return 0;
+}
+
+// Runs up to kMaxPivotJobs wedge-class jobs on one graph in one pivot-schedule launch.
+int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
+                           std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  hipStream_t stream = (hipStream_t)stream_v;
+  const int nj = (int)jobs.size();
+  out->assign(nj, JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (nj == 0) return 0;
+  if (nj > dev::kMaxPivotJobs) { if (err) *err = "too many jobs for one pivot launch"; return -7; }
+  int rc = device_derived_build(g, stream_v, nullptr, err);
+  if (rc) return rc;
+  bool need_c3 = false;
+  for (auto& j : jobs) {
+    j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m;
+    need_c3 = need_c3 || (j.kind == kWedgeGeneral);
+  }
+  // work items: pivots split into centre ranges of bounded walk cost
+  const long long budget = std::max<long long>(1 << 16, (long long)t.chunk << 12);
+  std::vector<dev::PivotItem> items;
+  std::vector<long long> icost;
+  for (int w = 0; w < g->n; w++) {
+    const int dw = g->h_idx[w + 1] - g->h_idx[w];
+    if (dw == 0) continue;
+    const long long c = g->h_piv_cost[w];
+    const long long k = std::max<long long>(1, std::min<long long>(dw, c / std::max(budget, c / 8 + 1)));
+    for (long long s = 0; s < k; s++) {
+      const int p0 = (int)(dw * s / k), p1 = (int)(dw * (s + 1) / k);
+      if (p1 > p0) { items.push_back(dev::PivotItem{w, p0, p1, 0}); icost.push_back(c / k + c / 2); }
+    }
+  }
+  // heaviest first
+  std::vector<int> order(items.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
+  std::vector<dev::PivotItem> sorted(items.size());
+  for (size_t i = 0; i < order.size(); i++) sorted[i] = items[order[i]];
+
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int grid = t.grid > 0 ? t.grid : cus * 4;
+  grid = std::max(1, std::min<int>(grid, (int)sorted.size()));
+  const long long dstride = ((long long)g->n + 63) / 64 * 64;
+  const long long estride = ((long long)g->maxdeg + 63) / 64 * 64;
+  unsigned* dbuf = nullptr;
+  unsigned* ebuf = nullptr;
+  Accum* d_acc = nullptr;
+  JobDesc* d_jobs = nullptr;
+  dev::PivotItem* d_items = nullptr;
+  HIP_TRY(hipMalloc((void**)&dbuf, sizeof(unsigned) * dstride * grid));
+  HIP_TRY(hipMalloc((void**)&ebuf, sizeof(unsigned) * estride * grid * dev::kPivWaves));
+  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
+  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(hipMalloc((void**)&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
+  HIP_TRY(hipMemsetAsync(dbuf, 0, sizeof(unsigned) * dstride * grid, stream));
+  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
+  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice, stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  const int nitems = (int)sorted.size();
+  const int c3 = need_c3 ? 1 : 0;
+  bool full = false;  // any fringe vector with a multi-vertex anchor set
+  for (auto& j : jobs)
+    if (j.kind == kWedgeGeneral)
+      for (int r = 0; r < j.rot; r++)
+        full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+  if (nitems > 0) {
+#define FSGC_PIVOT_LAUNCH(NJ, FULL)                                                                             \
+  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL>), dim3(grid), dim3(dev::kPivBlock), 0, stream, g->idx, g->adj, \
+                     g->tsup, g->tl_off, g->tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
+    if (nj == 1) {
+      if (full) FSGC_PIVOT_LAUNCH(1, true); else FSGC_PIVOT_LAUNCH(1, false);
+    } else {
+      if (full) FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, true); else FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, false);
+    }
+#undef FSGC_PIVOT_LAUNCH
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(e1, stream));
+  std::vector<Accum> h_acc(nj);
+  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_items);
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_acc);
+  (void)hipFree(ebuf);
+  (void)hipFree(dbuf);
+  for (int k = 0; k < nj; k++) {
+    JobResult& r = (*out)[k];
+    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
+    r.matches = h_acc[k].matches;
+    r.flags = h_acc[k].flags;
+  }
+  return 0;
+}
+
+}  // namespace fsgc

@@ -37,7 +37,7 @@ class _Result(C.Structure):
 class Tune(C.Structure):
     _fields_ = [
         ("block", C.c_int32), ("grid", C.c_int32), ("chunk", C.c_int32), ("hub_degree", C.c_int32),
-        ("lds_cap", C.c_int32), ("slice_mod", C.c_int32), ("slice_rem", C.c_int32), ("reserved", C.c_int32),
+        ("lds_cap", C.c_int32), ("slice_mod", C.c_int32), ("slice_rem", C.c_int32), ("schedule", C.c_int32),
     ]
 
 

@@ -65,7 +65,7 @@ typedef struct fsgc_tune {
   int32_t lds_cap;     /* longest neighbour list staged in LDS; 0 = default */
   int32_t slice_mod;   /* > 1: count only roots u with u % slice_mod == slice_rem */
   int32_t slice_rem;
-  int32_t reserved;
+  int32_t schedule;    /* 0: fastest schedule per class; 1: per-root schedule (the reference's) for every class */
 } fsgc_tune;
 
 /* Selects the GPU (the reference hard-codes device 0, fringeCount.cu:2276). */
