@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the Fringe-SGC counting path on one MI355X.
+
+Workload (default): R-MAT scale 20, edge factor 16, (a, b, c) = (0.57, 0.19, 0.19),
+seed 7, symmetrised and deduplicated (1,048,576 vertices, 31,400,418 directed
+entries), resident in HBM; motif: wedge core with 3 fringe vertices on the centre and
+one on each endpoint (`8 0 1 1 2 1 3 1 4 1 5 0 6 2 7`). One step = one exact count
+of that motif over the whole graph (fsgc_count; the per-graph edge index is built
+once, before the timed region, and reported as index_build_ms).
+
+Metric: core matches per second (core embeddings whose fringe formula is evaluated).
+Also reported: the roofline of the dominant kernel (k_wedge_pivot) from its
+algorithmic bytes and its HIP-event time, and a CPU baseline: the oracle (the
+reference's core+fringe method restated in C++/OpenMP) on a deterministic slice of
+root vertices, checked bit-exact against the GPU on the same slice.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fringe-sgc_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "c2": {
+        "spec": "rmat:20:16:0.57:0.19:0.19:7",
+        "motif": (8, [(0, 1), (1, 2), (1, 3), (1, 4), (1, 5), (0, 6), (2, 7)]),
+        "name": "R-MAT s20 ef16 (0.57,0.19,0.19) seed 7; wedge core + 3 centre / 1+1 endpoint fringes",
+        "cpu_slice": (4096,),
+    },
+    "c1": {
+        "spec": "er:4096:32768:1",
+        "motif": (5, [(0, 1), (1, 2), (0, 2), (0, 3), (0, 4)]),
+        "name": "ER G(4096, 32768) seed 1; triangle + 2 fringes on one core vertex",
+        "cpu_slice": (1,),
+    },
+    "c4": {
+        "spec": "chunglu:4000000:2.1:20:100000:19",
+        "motif": (9, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (0, 4), (0, 5), (0, 6), (1, 7), (1, 8)]),
+        "name": "Chung-Lu n=4e6 gamma 2.1 mean 20 max 1e5 seed 19; 4-clique + 3/2 fringes",
+        "cpu_slice": (4096,),
+    },
+}
+
+
+def physical_cores() -> int:
+    try:
+        import psutil
+        c = psutil.cpu_count(logical=False)
+        if c:
+            return int(c)
+    except Exception:
+        pass
+    return os.cpu_count() or 1
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return float(d["traffic_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the count shards by root vertices with one 16-byte reduction; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import fsgc
+    W = WORKLOADS[args.workload]
+    fsgc.init(0)
+    t0 = time.perf_counter()
+    ni, nl = fsgc.generate_csr(W["spec"])
+    g = fsgc.Graph.from_csr(ni, nl)
+    gen_s = time.perf_counter() - t0
+    n, m, maxdeg = g.info()
+    mo, _ = fsgc.preprocess(*W["motif"])
+    print(f"# graph {W['spec']}: n={n} m={m} maxdeg={maxdeg} (generated+uploaded in {gen_s:.1f}s)", flush=True)
+
+    # index build + automorphism sum (cached afterwards), untimed
+    first = g.count(mo)
+    print(f"# first call: index {first.index_ms:.1f} ms, kernel {first.kernel_ms:.1f} ms, class {first.class_name}, "
+          f"exact {first.exact}", flush=True)
+    for i in range(args.warmup):
+        r = g.count(mo)
+        print(f"# warmup {i}: kernel {r.kernel_ms:.1f} ms", flush=True)
+    kms, results = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = g.count(mo)
+        kms.append(r.kernel_ms)
+        results.append(r)
+        print(f"# step {i}: kernel {r.kernel_ms:.1f} ms wall {r.wall_ms:.1f} ms", flush=True)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1000.0
+    r = results[-1]
+    assert all(x.exact == r.exact and x.sg == r.sg for x in results), "counts differ between steps"
+    value = r.core_matches / (ms_per_step / 1000.0)
+    kernel_ms = sum(kms) / len(kms)
+    achieved = r.alg_bytes / (kernel_ms / 1000.0) / 1e9 if r.alg_bytes else None
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
+        "traffic": load_traffic("k_wedge_pivot", args.workload),
+        "kernel": "k_wedge_pivot" if r.motif_class == fsgc.CLASS_NAMES.index("wedge") else "k_count",
+        "kernel_ms": round(kernel_ms, 3),
+        "alg_bytes_per_launch": r.alg_bytes,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        # The oracle (the reference's per-root schedule on the host cores) on a
+        # deterministic sample: root vertices u with u % mod == rem for rem = mod-1,
+        # mod-2, ... until about cpu_budget seconds of CPU work; every sub-slice is also
+        # counted on the GPU (per-root schedule) and compared bit-exact.
+        import oracle
+        mod, budget = W["cpu_slice"][0], args.cpu_budget
+        cores = physical_cores()
+        done, matches, secs, parity = [], 0, 0.0, True
+        rem = mod - 1
+        while rem >= 0:
+            o = oracle.count(mo, ni, nl, threads=cores, slice_mod=mod, slice_rem=rem)
+            gs = g.count(mo, fsgc.Tune(slice_mod=mod, slice_rem=rem, schedule=1)) if mod > 1 else r
+            ok = (gs.sg == o["sg"] and gs.core_matches == o["core_matches"])
+            parity = parity and ok
+            done.append(rem)
+            matches += o["core_matches"]
+            secs += o["seconds"]
+            print(f"# cpu slice u % {mod} == {rem}: {o['core_matches']} matches in {o['seconds']:.2f} s "
+                  f"({o['threads']} threads), GPU identical: {ok}", flush=True)
+            if secs >= budget or mod == 1:
+                break
+            rem -= 1
+        cpu = {
+            "value": round(matches / secs, 1) if secs > 0 else None,
+            "unit": "core_matches/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": (f"roots u % {mod} in [{done[-1]}, {done[0]}] ({len(done)} slices, {matches} core matches, "
+                       f"{secs:.2f} s; reference per-root schedule, OpenMP); GPU slice sums identical: {parity}"),
+        }
+        if not parity:
+            print("# CPU/GPU slice mismatch!", file=sys.stderr)
+
+    out = {
+        "metric": "core_matches_per_s",
+        "value": round(value, 1),
+        "unit": "core_matches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u128",
+        "data": "synthetic",
+        "config": {
+            "workload": W["name"],
+            "graph": W["spec"],
+            "nodes": n,
+            "directed_edges": m,
+            "core_matches": r.core_matches,
+            "exact_count": str(r.exact),
+            "ref_compat_u64": r.ref_compat,
+            "flags": r.flags,
+            "index_build_ms": round(first.index_ms, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -307,6 +307,8 @@ int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* 
     finish(ps[i], res[i], g->dg.n, g->dg.m, &out[i]);
     out[i].kernel_ms = ms;
     out[i].wall_ms = wall;
+    out[i].alg_bytes = (double)res[i].bytes;
+    out[i].index_ms = res[i].index_ms;
   }
   if (batch_ms) *batch_ms = ms;
   return 0;

@@ -39,6 +39,7 @@ struct Accum {
   uint64_t matches;
   uint32_t flags;
   uint32_t pad;
+  uint64_t bytes;  // algorithmic global-memory bytes of the launch's schedule (pivot kernel)
 };
 
 struct JobDesc {

@@ -47,6 +47,8 @@ struct JobResult {
   u128 sum = 0;
   uint64_t matches = 0;
   uint32_t flags = 0;
+  uint64_t bytes = 0;     // algorithmic bytes of the launch (pivot schedule)
+  double index_ms = 0;    // per-graph index build time spent in this call
 };
 
 // pivot.hip

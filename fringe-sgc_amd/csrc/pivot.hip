@@ -229,6 +229,9 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
   unsigned* __restrict__ Eg = ebuf + ((long long)blockIdx.x * kPivWaves + wave) * estride;
   const G g{idx, adj};
   LaneAcc acc[NJ];
+  // algorithmic global-memory bytes of this schedule: every logical load, store or
+  // atomic on a global array counted at its size (atomics twice), no cache credit
+  ull bytes = 0;
 
   for (;;) {
     if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; }
@@ -244,7 +247,8 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
       const int xb = g.beg(x), xe = g.end(x);
       if (lane == 0 && x > w) atomicOr(&D[x], 1u);
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      for (int k = s + lane; k < xe; k += kWave) atomicAdd(&D[adj[k]], 2u);
+      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 8 : 0);
+      for (int k = s + lane; k < xe; k += kWave) { atomicAdd(&D[adj[k]], 2u); bytes += 12; }
     }
     __syncthreads();
 
@@ -263,6 +267,7 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
       bool in_slice = true;
       if (jobs[0].smod > 1) in_slice = (u % jobs[0].smod) == jobs[0].srem;
       if (!in_slice) continue;
+      if (lane == 0) bytes += 16 + 4 * (32 - __clz(du + 1));
       const bool lds = du <= kECap;
       unsigned* __restrict__ E = lds ? s_e[wave] : Eg;
       const bool c3on = need_c3 && tuw > 0;
@@ -285,6 +290,7 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
             while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] <= rq) lo = mid + 1; else hi = mid; }
             lb = lo;
           }
+          if (xq >= 0) bytes += 20 + 8 * (64 - __clzll(le - lb + 1)) + (le - lb) * (lds ? 4 : 12);
           const bool big = (le - lb) > 64;
           if (!big) {
             for (long long k = lb; k < le; k++) atomicAdd(&E[tl[k]], 1u);
@@ -311,6 +317,7 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
           const int tuv = tsup[ub + q];
           const int c3 = c3on ? (int)E[q] : 0;
           const int dv = g.deg(v);
+          bytes += 20 + ((c3on && !lds) ? 4 : 0);
 #pragma unroll
           for (int j = 0; j < NJ; j++) {
             if (j < njobs) {
@@ -333,13 +340,17 @@ __global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict
       const int xb = g.beg(x), xe = g.end(x);
       if (lane == 0 && x > w) D[x] = 0u;
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      for (int k = s + lane; k < xe; k += kWave) D[adj[k]] = 0u;
+      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 4 : 0);
+      for (int k = s + lane; k < xe; k += kWave) { D[adj[k]] = 0u; bytes += 8; }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int j = 0; j < NJ; j++)
     if (j < njobs) flush(acc[j], jobs[j].out);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bytes += __shfl_xor(bytes, o, kWave);
+  if (lane == 0) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
 // cost of pivot w: sum of the degrees of its neighbours
@@ -444,7 +455,8 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
   if (kernel_ms) *kernel_ms = 0;
   if (nj == 0) return 0;
   if (nj > dev::kMaxPivotJobs) { if (err) *err = "too many jobs for one pivot launch"; return -7; }
-  int rc = device_derived_build(g, stream_v, nullptr, err);
+  double index_ms = 0;
+  int rc = device_derived_build(g, stream_v, &index_ms, err);
   if (rc) return rc;
   bool need_c3 = false;
   for (auto& j : jobs) {
@@ -537,6 +549,8 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
     r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
     r.matches = h_acc[k].matches;
     r.flags = h_acc[k].flags;
+    r.bytes = h_acc[k].bytes;
+    r.index_ms = (k == 0) ? index_ms : 0;
   }
   return 0;
 }

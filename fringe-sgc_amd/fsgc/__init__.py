@@ -30,7 +30,7 @@ class _Result(C.Structure):
         ("exact_lo", C.c_uint64), ("exact_hi", C.c_uint64), ("ref_compat", C.c_uint64),
         ("sg_lo", C.c_uint64), ("sg_hi", C.c_uint64), ("aut", C.c_uint64),
         ("core_matches", C.c_uint64), ("flags", C.c_uint32), ("motif_class", C.c_int32),
-        ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
+        ("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("alg_bytes", C.c_double), ("index_ms", C.c_double),
     ]
 
 
@@ -97,6 +97,8 @@ class Result:
     motif_class: int
     kernel_ms: float
     wall_ms: float
+    alg_bytes: float = 0.0
+    index_ms: float = 0.0
 
     @property
     def class_name(self) -> str:
@@ -106,7 +108,7 @@ class Result:
 def _result(r: _Result) -> Result:
     return Result(exact=r.exact_lo | (r.exact_hi << 64), ref_compat=r.ref_compat, sg=r.sg_lo | (r.sg_hi << 64),
                   aut=r.aut, core_matches=r.core_matches, flags=r.flags, motif_class=r.motif_class,
-                  kernel_ms=r.kernel_ms, wall_ms=r.wall_ms)
+                  kernel_ms=r.kernel_ms, wall_ms=r.wall_ms, alg_bytes=r.alg_bytes, index_ms=r.index_ms)
 
 
 def init(device: int = 0):

@@ -55,6 +55,8 @@ typedef struct fsgc_result {
   int32_t motif_class;         /* FSGC_CLASS_* */
   double kernel_ms;            /* device time of the counting launch on the data graph */
   double wall_ms;              /* host wall time of the whole call */
+  double alg_bytes;            /* algorithmic global-memory bytes of the dominant launch's schedule (0 if not counted) */
+  double index_ms;             /* device time spent building the graph's per-edge index in this call (0 if cached) */
 } fsgc_result;
 
 typedef struct fsgc_tune {
