@@ -137,6 +137,19 @@ void finish(const Prepared& p, const fsgc::JobResult& r, int32_t n, int64_t m, f
   out->flags = flags;
 }
 
+enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2 };
+
+// Which kernel counts a motif on the data graph: the pivot schedule for wedge cores,
+// the rank schedule for triangle cores and the triangle kinds of the edge core, the
+// per-root persistent kernel otherwise (or for everything with schedule == 1).
+Schedule schedule_of(const Prepared& p, bool fast)
+{
+  if (!fast) return kPersistent;
+  if (p.job.cls == fsgc::kWedge) return kPivot;
+  if (fsgc::triangle_schedule_ok(p.job)) return kRank;
+  return kPersistent;
+}
+
 // Runs the data-graph jobs of all prepared motifs, plus the motif-graph jobs of the
 // motifs whose automorphism sum is not cached yet, in one persistent launch.
 int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune, std::vector<fsgc::JobResult>* res,
@@ -162,7 +175,7 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
       jobs.push_back(hj);
       graphs.push_back(&mgs[i]);
     }
-    if (!p.trivial && !(pivot && p.job.cls == fsgc::kWedge)) {
+    if (!p.trivial && schedule_of(p, pivot) == kPersistent) {
       data_job[i] = (int)jobs.size();
       jobs.push_back(p.job);
       graphs.push_back(&g->dg);
@@ -172,19 +185,33 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   double ms_total = 0, ms = 0;
   if (!rc) rc = fsgc::device_run_jobs(jobs, graphs, to_tune(tune), nullptr, &jr, &ms, &err);
   ms_total += ms;
-  // wedge-class motifs on the data graph: pivot schedule, up to kMaxPivotJobs per launch
-  std::vector<fsgc::JobResult> wres(ps.size());
-  if (!rc && pivot) {
-    std::vector<size_t> wl;
-    for (size_t i = 0; i < ps.size(); i++)
-      if (!ps[i].trivial && ps[i].job.cls == fsgc::kWedge) wl.push_back(i);
-    for (size_t b = 0; b < wl.size() && !rc; b += fsgc::kMaxPivotJobs) {
+  // wedge-class motifs: pivot schedule; triangle-based motifs: rank schedule; each
+  // launch takes up to 8 motifs that share one enumeration
+  std::vector<fsgc::JobResult> sres(ps.size());
+  for (int sched : {kPivot, kRank}) {
+    if (rc || !pivot) break;
+    std::vector<std::vector<size_t>> groups;
+    for (size_t i = 0; i < ps.size(); i++) {
+      if (ps[i].trivial || schedule_of(ps[i], pivot) != sched) continue;
+      // tailed 4-cliques with different tail counts go to separate launches
+      const int key = (ps[i].job.cls == fsgc::kTriangle && ps[i].job.kind == fsgc::kTriTailedClique) ? ps[i].job.p[3] : -1;
+      bool placed = false;
+      for (auto& gr : groups) {
+        const auto& j0 = ps[gr[0]].job;
+        const int k0 = (j0.cls == fsgc::kTriangle && j0.kind == fsgc::kTriTailedClique) ? j0.p[3] : -1;
+        if ((int)gr.size() < fsgc::kMaxPivotJobs && (key < 0 || k0 < 0 || key == k0)) { gr.push_back(i); placed = true; break; }
+      }
+      if (!placed) groups.push_back({i});
+    }
+    for (auto& gr : groups) {
       std::vector<fsgc::JobDesc> wj;
-      for (size_t k = b; k < std::min(wl.size(), b + fsgc::kMaxPivotJobs); k++) wj.push_back(ps[wl[k]].job);
+      for (size_t k : gr) wj.push_back(ps[k].job);
       std::vector<fsgc::JobResult> r;
-      rc = fsgc::device_run_wedge_pivot(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      rc = (sched == kPivot) ? fsgc::device_run_wedge_pivot(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err)
+                             : fsgc::device_run_triangles(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      if (rc) break;
       ms_total += ms;
-      for (size_t k = 0; k < r.size(); k++) wres[wl[b + k]] = r[k];
+      for (size_t k = 0; k < r.size(); k++) sres[gr[k]] = r[k];
     }
   }
   if (kernel_ms) *kernel_ms = ms_total;
@@ -193,7 +220,7 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   res->assign(ps.size(), fsgc::JobResult());
   for (size_t i = 0; i < ps.size(); i++) {
     if (data_job[i] >= 0) (*res)[i] = jr[data_job[i]];
-    else if (pivot && !ps[i].trivial && ps[i].job.cls == fsgc::kWedge) (*res)[i] = wres[i];
+    else if (!ps[i].trivial) (*res)[i] = sres[i];
     if (aut_job[i] >= 0) {
       Prepared& p = ps[i];
       p.aut.sum = jr[aut_job[i]].sum;

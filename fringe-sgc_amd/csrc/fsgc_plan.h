@@ -58,6 +58,12 @@ constexpr int kMaxPivotJobs = 8;
 int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                            std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
+// tri.hip
+constexpr int kMaxTriJobs = 8;
+bool triangle_schedule_ok(const JobDesc& j);
+int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                         std::vector<JobResult>* out, double* kernel_ms, std::string* err);
+
 // kernels.hip
 int device_init(int device, std::string* err);
 int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err);

@@ -1,0 +1,326 @@
+// Triangle-based counting on the rank schedule (gfx950).
+//
+// Every triangle {x, y, z} is visited once, from its vertex of lowest rank
+// (rank = (degree, id)): for x, each higher-ranked neighbour y, and each common
+// neighbour z of x and y ranked above y. The per-edge common-neighbour lists T(x, .)
+// (positions in N(x), built once per graph, see pivot.hip) give
+//   t_xy, t_xz from the support array, t_yz by one search in N(y), and
+//   c3 = |N(x) & N(y) & N(z)| = |T(x,y) & T(x,z)|: T(x,y) is marked in LDS (indexed by
+//   position in N(x)) and each lane sums the marks over T(x,z) for its own z.
+// Since x has the lowest degree of the three, the lists walked are the shortest ones.
+// The triangle is then relabelled to the reference's roles (u > v > w by id) and the
+// reference's filters and formulas are applied: triangle cores FC:967-1062
+// (general), FC:774-931 (4-clique and tailed 4-clique), and the triangle kernels of
+// the edge core FC:1864-1966.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "devmath.h"
+#include "fsgc_plan.h"
+
+namespace fsgc {
+namespace dev {
+
+constexpr int kTriBlock = 256;
+constexpr int kTriWaves = kTriBlock / kWave;
+constexpr int kMarkCap = 8192;  // LDS marks per wave (positions of N(x))
+constexpr int kMaxTriJobs = 8;
+
+struct TriItem {
+  int32_t x;
+  int32_t p0;  // positions [p0, p1) of N(x) taken as y
+  int32_t p1;
+  int32_t pad;
+};
+
+__device__ __forceinline__ bool rank_less(int da, int a, int db, int b) { return da < db || (da == db && a < b); }
+
+__device__ __forceinline__ int lbound(const int* __restrict__ a, int l, int r, int x)
+{
+  while (l < r) {
+    const int mid = (l + r) >> 1;
+    if (a[mid] < x) l = mid + 1; else r = mid;
+  }
+  return l;
+}
+
+template <bool kFull>
+__device__ __forceinline__ u128 fc3_tri(const int* fr, const int* venn, uint32_t& fl)
+{
+  if constexpr (kFull) {
+    if (!fc3_is_singleton(fr)) return fc3(fr, venn, fl);
+  }
+  return fc3_singletons(fr, venn, fl);
+}
+
+// One triangle in reference roles (u > v > w by id) for one job. `c3lt` is the number
+// of common neighbours q < w, `wq` the sum over them of C(d_q - 3, tails).
+template <bool kFull>
+__device__ __forceinline__ bool eval_triangle(const JobDesc* __restrict__ J, int du, int dv, int dw, int tuv, int tuw,
+                                              int tvw, int c3, int c3lt, u128 wq, u128& val, uint32_t& fl)
+{
+  const int* p = J->p;
+  if (J->cls == kEdge) {  // FC:2054-2103 filters, then FC:1864-1966
+    if (!(du > 1 && dv > 1)) return false;
+    if (J->kind == kEdgeTriangle) { val = 1; return true; }
+    const int t = p[2];
+    val = addx(addx(ncr(du - 2, t, fl), ncr(dv - 2, t, fl), fl), ncr(dw - 2, t, fl), fl);
+    return true;
+  }
+  const int sum1 = p[0], sum2 = p[1], sum3 = p[2], tails = p[3];
+  if (du < sum1 || dv < max(sum1, sum2 - du)) return false;  // FC:1167-1218
+  if (J->kind != kTriGeneral) {
+    if (dw < 3) return false;
+    if (J->kind == kTriClique) { val = (u128)(unsigned)c3lt; return true; }
+    const u128 c123 = addx(addx(ncr(du - 3, tails, fl), ncr(dv - 3, tails, fl), fl), ncr(dw - 3, tails, fl), fl);
+    val = addx(mulx(c123, (u128)(unsigned)c3lt, fl), wq, fl);
+    return true;
+  }
+  const int thres2 = max(max(sum1, sum3 - du - dv), max(sum2 - du, sum2 - dv));
+  if (dw < thres2) return false;
+  int venn[8];  // FC:948-962
+  const int cc12 = tuv - (c3 + 1), c13 = tuw - (c3 + 1), c23 = tvw - (c3 + 1);
+  venn[0] = 0;
+  venn[iuvw] = c3;
+  venn[iuv] = cc12;
+  venn[iuw] = c13;
+  venn[ivw] = c23;
+  venn[iu] = du - (cc12 + c13 + (c3 + 2));
+  venn[iv] = dv - (cc12 + c23 + (c3 + 2));
+  venn[iw] = dw - (c13 + c23 + (c3 + 2));
+  u128 s = 0;
+  for (int q = 0; q < J->rot; q++) s = addx(s, fc3_tri<kFull>(J->fr[q], venn, fl), fl);
+  val = s;
+  return true;
+}
+
+template <int NJ, bool kFull>
+__global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
+                                                             const int* __restrict__ tsup,
+                                                             const long long* __restrict__ tl_off,
+                                                             const int* __restrict__ tl,
+                                                             const JobDesc* __restrict__ jobs, int njobs, int need_c3,
+                                                             int need_lt, int tails, const TriItem* __restrict__ items,
+                                                             int nitems,
+                                                             int* __restrict__ head, unsigned char* __restrict__ gmark,
+                                                             long long mstride)
+{
+  __shared__ unsigned char s_mark[kTriWaves][kMarkCap];
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  unsigned char* __restrict__ Mg = gmark + ((long long)blockIdx.x * kTriWaves + wave) * mstride;
+  const G g{idx, adj};
+  LaneAcc acc[NJ];
+  const int smod = jobs[0].smod, srem = jobs[0].srem;
+  // LDS holds whatever the previous kernel left: clear this wave's marks once
+  for (int i = lane; i < kMarkCap; i += kWave) s_mark[wave][i] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(head, 1);
+    it = bcast(it, 0);
+    if (it >= nitems) break;
+    const TriItem I = items[it];
+    const int x = I.x, xb = g.beg(x), dx = g.deg(x);
+    const bool lds = dx <= kMarkCap;
+    unsigned char* __restrict__ M = lds ? s_mark[wave] : Mg;
+    for (int py = I.p0; py < I.p1; py++) {
+      const int y = adj[xb + py];
+      const int dy = g.deg(y);
+      if (!rank_less(dx, x, dy, y)) continue;
+      const long long yb = tl_off[xb + py], ye = tl_off[xb + py + 1];
+      if (ye - yb < 1) continue;
+      const int txy = tsup[xb + py];
+      // mark T(x, y)
+      if (need_c3) {
+        for (long long k = yb + lane; k < ye; k += kWave) M[tl[k]] = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      // lanes over z in T(x, y) ranked above y
+      for (long long k0 = yb; k0 < ye; k0 += kWave) {
+        const long long k = k0 + lane;
+        if (k < ye) {
+          const int pz = tl[k];
+          const int z = adj[xb + pz];
+          const int dz = g.deg(z);
+          if (rank_less(dy, y, dz, z)) {
+            const int txz = tsup[xb + pz];
+            const int yb2 = g.beg(y);
+            const int pyz = lbound(adj, yb2, g.end(y), z);
+            const int tyz = tsup[pyz];
+            // reference roles by id: u > v > w
+            int a = x, da = dx, b = y, db = dy, c = z, dc = dz;
+            int tab = txy, tac = txz, tbc = tyz;
+            // sort (a, b, c) descending by id, carrying degrees and supports
+            if (a < b) { int t = a; a = b; b = t; t = da; da = db; db = t; t = tac; tac = tbc; tbc = t; }
+            if (b < c) { int t = b; b = c; c = t; t = db; db = dc; dc = t; t = tab; tab = tac; tac = t; }
+            if (a < b) { int t = a; a = b; b = t; t = da; da = db; db = t; t = tac; tac = tbc; tbc = t; }
+            const int u = a;
+            if (smod > 1 && (u % smod) != srem) continue;
+            int c3 = 0, c3lt = 0;
+            u128 wq = 0;
+            uint32_t fl0 = 0;
+            if (need_c3) {
+              const long long zb = tl_off[xb + pz], ze = tl_off[xb + pz + 1];
+              const int wpos = need_lt ? lbound(adj, xb, xb + dx, c) - xb : 0;  // positions below w = c
+              for (long long q = zb; q < ze; q++) {
+                const int pq = tl[q];
+                if (M[pq]) {
+                  c3++;
+                  if (need_lt && pq < wpos) {
+                    c3lt++;
+                    if (tails) wq = addx(wq, ncr(g.deg(adj[xb + pq]) - 3, tails, fl0), fl0);
+                  }
+                }
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+              if (j < njobs) {
+                u128 val;
+                if (eval_triangle<kFull>(jobs + j, da, db, dc, tab, tac, tbc, c3, c3lt, wq, val, acc[j].fl)) {
+                  acc[j].add(val);
+                  acc[j].matches++;
+                  acc[j].fl |= fl0;
+                }
+              }
+            }
+          }
+        }
+      }
+      // unmark
+      if (need_c3) {
+        __builtin_amdgcn_wave_barrier();
+        for (long long k = yb + lane; k < ye; k += kWave) M[tl[k]] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; j++)
+    if (j < njobs) flush(acc[j], jobs[j].out);
+}
+
+}  // namespace dev
+
+#define HIP_TRY(expr)                                                    \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) {                                              \
+      if (err) *err = std::string(#expr) + ": " + hipGetErrorString(e_); \
+      return -5;                                                         \
+    }                                                                    \
+  } while (0)
+
+bool triangle_schedule_ok(const JobDesc& j)
+{
+  return j.cls == kTriangle || (j.cls == kEdge && (j.kind == kEdgeTriangle || j.kind == kEdgeTailed));
+}
+
+// Runs up to kMaxTriJobs triangle-based jobs (same graph) in one rank-schedule launch.
+int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
+                         std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  hipStream_t stream = (hipStream_t)stream_v;
+  const int nj = (int)jobs.size();
+  out->assign(nj, JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (nj == 0) return 0;
+  if (nj > dev::kMaxTriJobs) { if (err) *err = "too many jobs for one triangle launch"; return -7; }
+  int tails = -1;
+  for (auto& j : jobs) {
+    if (j.cls == kTriangle && j.kind == kTriTailedClique) {
+      if (tails >= 0 && tails != j.p[3]) { if (err) *err = "tailed 4-clique jobs with different tails"; return -22; }
+      tails = j.p[3];
+    }
+  }
+  double index_ms = 0;
+  int rc = device_derived_build(g, stream_v, &index_ms, err);
+  if (rc) return rc;
+  bool need_c3 = false, need_lt = false, full = false;
+  for (auto& j : jobs) {
+    j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m;
+    if (j.cls == kTriangle) {
+      need_c3 = true;
+      if (j.kind != kTriGeneral) need_lt = true;
+      for (int r = 0; r < j.rot; r++) full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+    }
+  }
+  // items: vertices x with their positions split into bounded pieces
+  std::vector<dev::TriItem> items;
+  const long long budget = std::max<long long>(64, t.chunk);
+  for (int x = 0; x < g->n; x++) {
+    const int dx = g->h_idx[x + 1] - g->h_idx[x];
+    if (dx < 2) continue;
+    const int per = (int)std::max<long long>(1, budget * 64 / std::max(64, dx));
+    for (int p = 0; p < dx; p += per) items.push_back(dev::TriItem{x, p, std::min(dx, p + per), 0});
+  }
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int grid = t.grid > 0 ? t.grid : cus * 4;
+  grid = std::max(1, std::min<int>(grid, ((int)items.size() + dev::kTriWaves - 1) / dev::kTriWaves));
+  const long long mstride = ((long long)g->maxdeg + 63) / 64 * 64;
+  unsigned char* gmark = nullptr;
+  Accum* d_acc = nullptr;
+  JobDesc* d_jobs = nullptr;
+  dev::TriItem* d_items = nullptr;
+  HIP_TRY(hipMalloc((void**)&gmark, (size_t)mstride * grid * dev::kTriWaves));
+  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
+  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(hipMalloc((void**)&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(hipMemsetAsync(gmark, 0, (size_t)mstride * grid * dev::kTriWaves, stream));
+  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
+  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice, stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  const int nitems = (int)items.size();
+  if (nitems > 0) {
+#define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
+  hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, stream, g->idx,     \
+                     g->adj, g->tsup, g->tl_off, g->tl, d_jobs, nj, need_c3 ? 1 : 0, need_lt ? 1 : 0,           \
+                     std::max(0, tails), d_items,                                                               \
+                     nitems, d_head, gmark, mstride)
+    if (nj == 1) {
+      if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
+    } else {
+      if (full) FSGC_TRI_LAUNCH(dev::kMaxTriJobs, true); else FSGC_TRI_LAUNCH(dev::kMaxTriJobs, false);
+    }
+#undef FSGC_TRI_LAUNCH
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(e1, stream));
+  std::vector<Accum> h_acc(nj);
+  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_items);
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_acc);
+  (void)hipFree(gmark);
+  for (int k = 0; k < nj; k++) {
+    JobResult& r = (*out)[k];
+    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
+    r.matches = h_acc[k].matches;
+    r.flags = h_acc[k].flags;
+    r.index_ms = (k == 0) ? index_ms : 0;
+  }
+  return 0;
+}
+
+}  // namespace fsgc

@@ -137,3 +137,20 @@ def test_wedge_pivot_schedule_matches_per_root(gpu):
                 continue
             same(a, o, f"pivot {gpu.live_fields(mo)} on {spec}")
             assert (b.sg, b.core_matches) == (a.sg, a.core_matches)
+
+
+def test_triangle_rank_schedule_matches_per_root(gpu):
+    """The rank schedule (default for triangle cores and the triangle kinds of edge cores)
+    agrees with the per-root schedule and with the oracle, cliques included."""
+    for spec in ("rmat:9:8:0.57:0.19:0.19:23", "er:500:3000:24", "chunglu:3000:2.1:12:300:25"):
+        ni, nl = gpu.generate_csr(spec)
+        g = gpu.Graph.from_csr(ni, nl)
+        mos = [m for m in gpu.random_motifs(3, 64) if m[0] in (2, 3)]
+        mos += [gpu.preprocess_pattern(p)[0] for p in (3, 6, 9, 12, 20, 21, 25, 27, 29)]
+        mos += [gpu.preprocess(*C4_MOTIF)[0], gpu.preprocess(*C1_MOTIF)[0]]
+        for mo in mos:
+            a = g.count(mo)
+            b = g.count(mo, gpu.Tune(schedule=1))
+            assert (a.sg, a.core_matches, a.exact) == (b.sg, b.core_matches, b.exact), (gpu.live_fields(mo), spec)
+            o = oracle.count(mo, ni, nl)
+            same(a, o, f"rank {gpu.live_fields(mo)} on {spec}")
