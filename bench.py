@@ -72,6 +72,47 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
+def batch_compare(spec, seed=3, count=64, max_core=3, out_path=None):
+    """Plain path (one fsgc_count per motif) against the batched path (one
+    fsgc_count_batch) on the same resident graph: motifs/s of each, and per-motif
+    equality of every result. Motifs with cores of 4-6 vertices are left out when
+    max_core < 4 (their reference enumeration is exponential in the core size)."""
+    import fsgc
+    fsgc.init(0)
+    g = fsgc.Graph.generate(spec)
+    mos_all = fsgc.random_motifs(seed, count)
+    mos = [m for m in mos_all if m[0] <= max_core]
+    # automorphism sums and the per-edge index are per motif / per graph: warm them once
+    t0 = time.perf_counter()
+    g.count_batch(mos)
+    print(f"# warm batch (index + automorphism sums) {time.perf_counter() - t0:.1f} s", flush=True)
+    plain, t_plain = [], 0.0
+    for i, mo in enumerate(mos):
+        t0 = time.perf_counter()
+        r = g.count(mo)
+        dt = time.perf_counter() - t0
+        t_plain += dt
+        plain.append(r)
+        print(f"# plain motif {i} ({r.class_name}): {dt:.2f} s", flush=True)
+    t0 = time.perf_counter()
+    batch, batch_ms = g.count_batch(mos)
+    t_batch = time.perf_counter() - t0
+    same = all((a.exact, a.sg, a.aut, a.ref_compat, a.core_matches) == (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches)
+               for a, b in zip(plain, batch))
+    res = {
+        "graph": spec, "motifs": len(mos), "motifs_generated": count, "seed": seed,
+        "excluded": f"{count - len(mos)} motifs with 4-6 core vertices",
+        "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
+        "plain_motifs_per_s": round(len(mos) / t_plain, 4), "batched_motifs_per_s": round(len(mos) / t_batch, 4),
+        "speedup": round(t_plain / t_batch, 3), "identical": same,
+        "classes": {n: sum(1 for r in plain if r.class_name == n) for n in fsgc.CLASS_NAMES},
+    }
+    print(json.dumps(res), flush=True)
+    if out_path:
+        json.dump({"summary": res, "counts": [str(r.exact) for r in batch]}, open(out_path, "w"), indent=1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +121,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    ap.add_argument("--batch", default=None, help="graph spec: compare plain vs batched over the random motifs")
+    ap.add_argument("--batch-out", default=None)
     args = ap.parse_args()
+    if args.batch:
+        batch_compare(args.batch, out_path=args.batch_out)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the count shards by root vertices with one 16-byte reduction; see DESIGN.md)", file=sys.stderr)

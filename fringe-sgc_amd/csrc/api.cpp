@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -137,7 +138,14 @@ void finish(const Prepared& p, const fsgc::JobResult& r, int32_t n, int64_t m, f
   out->flags = flags;
 }
 
-enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2 };
+enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2, kEdgePass = 3 };
+
+// FSGC_LOG=1 in the environment: one line per kernel launch on stderr
+bool log_on()
+{
+  static const bool on = [] { const char* e = getenv("FSGC_LOG"); return e && *e && *e != '0'; }();
+  return on;
+}
 
 // Which kernel counts a motif on the data graph: the pivot schedule for wedge cores,
 // the rank schedule for triangle cores and the triangle kinds of the edge core, the
@@ -147,6 +155,7 @@ Schedule schedule_of(const Prepared& p, bool fast)
   if (!fast) return kPersistent;
   if (p.job.cls == fsgc::kWedge) return kPivot;
   if (fsgc::triangle_schedule_ok(p.job)) return kRank;
+  if (fsgc::edgepass_schedule_ok(p.job)) return kEdgePass;
   return kPersistent;
 }
 
@@ -185,10 +194,11 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   double ms_total = 0, ms = 0;
   if (!rc) rc = fsgc::device_run_jobs(jobs, graphs, to_tune(tune), nullptr, &jr, &ms, &err);
   ms_total += ms;
+  if (log_on() && !jobs.empty()) fprintf(stderr, "[fsgc] per-root launch: %zu jobs, %.1f ms\n", jobs.size(), ms);
   // wedge-class motifs: pivot schedule; triangle-based motifs: rank schedule; each
   // launch takes up to 8 motifs that share one enumeration
   std::vector<fsgc::JobResult> sres(ps.size());
-  for (int sched : {kPivot, kRank}) {
+  for (int sched : {kPivot, kRank, kEdgePass}) {
     if (rc || !pivot) break;
     std::vector<std::vector<size_t>> groups;
     for (size_t i = 0; i < ps.size(); i++) {
@@ -207,10 +217,14 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
       std::vector<fsgc::JobDesc> wj;
       for (size_t k : gr) wj.push_back(ps[k].job);
       std::vector<fsgc::JobResult> r;
-      rc = (sched == kPivot) ? fsgc::device_run_wedge_pivot(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err)
-                             : fsgc::device_run_triangles(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      if (sched == kPivot) rc = fsgc::device_run_wedge_pivot(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      else if (sched == kRank) rc = fsgc::device_run_triangles(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
+      else rc = fsgc::device_run_edgepass(wj, &g->dg, to_tune(tune), nullptr, &r, &ms, &err);
       if (rc) break;
       ms_total += ms;
+      if (log_on())
+        fprintf(stderr, "[fsgc] %s launch: %zu motifs, %.1f ms\n",
+                sched == kPivot ? "pivot" : (sched == kRank ? "rank" : "edge-pass"), gr.size(), ms);
       for (size_t k = 0; k < r.size(); k++) sres[gr[k]] = r[k];
     }
   }

@@ -61,6 +61,9 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
 // tri.hip
 constexpr int kMaxTriJobs = 8;
 bool triangle_schedule_ok(const JobDesc& j);
+bool edgepass_schedule_ok(const JobDesc& j);
+int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                        std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                          std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 

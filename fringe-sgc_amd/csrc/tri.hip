@@ -207,6 +207,55 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
     if (j < njobs) flush(acc[j], jobs[j].out);
 }
 
+// One sweep over the directed CSR entries evaluating every edge-core motif of the
+// closed-form kind (FC:1999-2051: c_uv is the support t_uv, read from the index) and
+// every star motif (FC:2152-2169, evaluated at each vertex's first entry). Lanes take
+// consecutive entries; the owner of an entry comes from a search over the item's
+// vertex range.
+template <int NJ>
+__global__ __launch_bounds__(256) void k_edge_pass(const int* __restrict__ idx, const int* __restrict__ adj,
+                                                  const int* __restrict__ tsup, const JobDesc* __restrict__ jobs,
+                                                  int njobs, const WorkItem* __restrict__ items, int nitems,
+                                                  int* __restrict__ head)
+{
+  const int lane = lane_id();
+  const G g{idx, adj};
+  LaneAcc acc[NJ];
+  const int smod = jobs[0].smod, srem = jobs[0].srem;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(head, 1);
+    it = bcast(it, 0);
+    if (it >= nitems) break;
+    const WorkItem I = items[it];
+    for (long long j = I.begin + lane; j < I.end; j += kWave) {
+      const int u = owner_of(idx, I.u0, I.u1, j);
+      if (smod > 1 && (u % smod) != srem) continue;
+      const int v = adj[j];
+      const int du = g.deg(u), dv = g.deg(v);
+      const bool first = (j == idx[u]);
+      const int c = tsup[j];
+#pragma unroll
+      for (int k = 0; k < NJ; k++) {
+        if (k >= njobs) continue;
+        const JobDesc* J = jobs + k;
+        if (J->cls == kVertex) {
+          if (first && du >= J->p[0]) { acc[k].add(ncr(du, J->p[0], acc[k].fl)); acc[k].matches++; }
+        } else {
+          const int* p = J->p;  // f_u, f_uv, f_v, lim1, lim2 (FC:2054-2103 filters)
+          if (v < u && du > p[3] && dv > p[3] && du + dv > p[4]) {
+            acc[k].add(pair_formula(du, dv, c, 1, p[0], p[1], p[2], acc[k].fl));
+            acc[k].matches++;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NJ; k++)
+    if (k < njobs) flush(acc[k], jobs[k].out);
+}
+
 }  // namespace dev
 
 #define HIP_TRY(expr)                                                    \
@@ -221,6 +270,85 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
 bool triangle_schedule_ok(const JobDesc& j)
 {
   return j.cls == kTriangle || (j.cls == kEdge && (j.kind == kEdgeTriangle || j.kind == kEdgeTailed));
+}
+
+bool edgepass_schedule_ok(const JobDesc& j)
+{
+  return (j.cls == kEdge && j.kind == kEdgeGeneral) || (j.cls == kVertex && j.kind == kVertexStars);
+}
+
+// Runs up to kMaxTriJobs closed-form edge and star jobs in one sweep over the edges.
+int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
+                        std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  hipStream_t stream = (hipStream_t)stream_v;
+  const int nj = (int)jobs.size();
+  out->assign(nj, JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (nj == 0) return 0;
+  if (nj > dev::kMaxTriJobs) { if (err) *err = "too many jobs for one edge pass"; return -7; }
+  double index_ms = 0;
+  int rc = device_derived_build(g, stream_v, &index_ms, err);
+  if (rc) return rc;
+  for (auto& j : jobs) { j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m; }
+  std::vector<WorkItem> items;
+  const long long step = std::max<long long>(256, (long long)t.chunk * 32);
+  const std::vector<int32_t>& hi = g->h_idx;
+  for (long long b = 0; b < g->m; b += step) {
+    const long long e = std::min<long long>(g->m, b + step);
+    const int u0 = (int)(std::upper_bound(hi.begin(), hi.end(), (int32_t)b) - hi.begin()) - 1;
+    const int u1 = (int)(std::upper_bound(hi.begin(), hi.end(), (int32_t)(e - 1)) - hi.begin()) - 1;
+    items.push_back(WorkItem{0, u0, u1, 0, b, e});
+  }
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int grid = t.grid > 0 ? t.grid : cus * 8;
+  grid = std::max(1, std::min<int>(grid, ((int)items.size() + 3) / 4));
+  Accum* d_acc = nullptr;
+  JobDesc* d_jobs = nullptr;
+  WorkItem* d_items = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
+  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(hipMalloc((void**)&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
+  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, stream));
+  if (!items.empty()) {
+    if (nj == 1)
+      hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, stream, g->idx, g->adj, g->tsup, d_jobs, nj,
+                         d_items, (int)items.size(), d_head);
+    else
+      hipLaunchKernelGGL((dev::k_edge_pass<dev::kMaxTriJobs>), dim3(grid), dim3(256), 0, stream, g->idx, g->adj,
+                         g->tsup, d_jobs, nj, d_items, (int)items.size(), d_head);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(e1, stream));
+  std::vector<Accum> h_acc(nj);
+  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (kernel_ms) *kernel_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_items);
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_acc);
+  for (int k = 0; k < nj; k++) {
+    JobResult& r = (*out)[k];
+    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
+    r.matches = h_acc[k].matches;
+    r.flags = h_acc[k].flags;
+    r.index_ms = (k == 0) ? index_ms : 0;
+  }
+  return 0;
 }
 
 // Runs up to kMaxTriJobs triangle-based jobs (same graph) in one rank-schedule launch.
