@@ -1,0 +1,226 @@
+// Host side of a counting call (see count.h). The class dispatch, the automorphism
+// run on the motif graph and the final division follow occurrences() of the
+// reference (fringeCount.cu:2199-2262); which kernel runs each class is this
+// library's choice (pivot / rank / edge-pass schedules, or the per-root kernel).
+#include "count.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+
+#include "devscratch.h"
+
+namespace fsgc {
+
+namespace {
+std::mutex g_aut_mu;
+std::map<std::string, AutEntry> g_aut;
+}  // namespace
+
+const char* schedule_name(int s)
+{
+  switch (s) {
+    case kPersistent: return "per-root";
+    case kPivot: return "pivot";
+    case kRank: return "rank";
+    case kEdgePass: return "edge-pass";
+    default: return "?";
+  }
+}
+
+// FSGC_LOG=1 in the environment: one line per kernel launch on stderr
+bool log_on()
+{
+  static const bool on = [] { const char* e = getenv("FSGC_LOG"); return e && *e && *e != '0'; }();
+  return on;
+}
+
+int prepare(const uint8_t* mob, const fsgc_tune* tune, Prepared* p, std::string* err)
+{
+  std::memcpy(&p->mo, mob, sizeof(p->mo));
+  int rc = make_job(p->mo, &p->job, err);
+  if (rc) return rc;
+  if (tune && tune->slice_mod > 1) { p->job.smod = tune->slice_mod; p->job.srem = tune->slice_rem; }
+  p->trivial = (p->job.cls == kVertex && p->job.kind != kVertexStars);
+  p->need_aut = p->job.cls >= kWedge;
+  p->qf = general_quirk_factor(p->mo);
+  if (p->need_aut) {
+    // only the live fields determine the motif
+    MatchingOrder z;
+    std::memset(&z, 0, sizeof(z));
+    z.nodes = p->mo.nodes; z.anchorsets = p->mo.anchorsets; z.startdeg = p->mo.startdeg;
+    for (int i = 0; i + 1 < p->mo.nodes; i++) { z.destdeg[i] = p->mo.destdeg[i]; z.list[i] = p->mo.list[i]; }
+    for (int i = 0; i < p->mo.anchorsets; i++) { z.anch[i] = p->mo.anch[i]; z.fcnt[i] = p->mo.fcnt[i]; }
+    p->key.assign((const char*)&z, sizeof(z));
+    std::lock_guard<std::mutex> lk(g_aut_mu);
+    auto it = g_aut.find(p->key);
+    if (it != g_aut.end()) { p->aut = it->second; p->aut_cached = true; }
+    else { p->mgraph = motif_graph(p->mo); p->aut.hn = p->mgraph.n; p->aut.hm = p->mgraph.m; }
+  }
+  return 0;
+}
+
+void finish(const Prepared& p, const JobResult& r, int32_t n, int64_t m, fsgc_result* out)
+{
+  out->motif_class = p.job.cls;
+  out->core_matches = r.matches;
+  uint32_t flags = r.flags;
+  u128 sg = r.sum, aut = 1;
+  if (p.trivial) sg = (p.job.kind == kVertexCount) ? (u128)n : (u128)(m / 2);
+  if (p.need_aut) {
+    if (p.aut.hn > n || p.aut.hm > m) {  // FC:2216-2219: the motif cannot fit
+      std::memset(out, 0, offsetof(fsgc_result, kernel_ms));
+      out->motif_class = p.job.cls;
+      return;
+    }
+    aut = p.aut.sum * (u128)p.qf;
+    sg = sg * (u128)p.qf;
+    flags |= p.aut.flags;
+  }
+  out->sg_lo = (uint64_t)r.sum;
+  out->sg_hi = (uint64_t)(r.sum >> 64);
+  if (p.trivial) { out->sg_lo = (uint64_t)sg; out->sg_hi = (uint64_t)(sg >> 64); }
+  out->aut = (uint64_t)aut;
+  if (aut == 0) { out->exact_lo = out->exact_hi = out->ref_compat = 0; out->flags = flags | FSGC_AUT_REMAINDER; return; }
+  const u128 q = sg / aut;
+  if (p.job.smod <= 1 && (sg % aut) != 0) flags |= FSGC_AUT_REMAINDER;
+  if ((q >> 64) != 0) flags |= FSGC_OVF64;
+  out->exact_lo = (uint64_t)q;
+  out->exact_hi = (uint64_t)(q >> 64);
+  // fringeCount.cu:2213, 2230: the automorphism sum passes through an int
+  const uint64_t ref_aut = (uint64_t)(int64_t)(int32_t)(uint32_t)(uint64_t)aut;
+  if ((u128)ref_aut != aut) flags |= FSGC_REF_AUT_TRUNC;
+  out->ref_compat = ref_aut ? (uint64_t)sg / ref_aut : 0;
+  out->flags = flags;
+}
+
+// The pivot schedule for wedge cores, the rank schedule for triangle cores and the
+// triangle kinds of the edge core, one edge pass for the other edge kinds and the
+// stars, the per-root persistent kernel otherwise (or for everything when !fast).
+Schedule schedule_of(const Prepared& p, bool fast)
+{
+  if (!fast) return kPersistent;
+  if (p.job.cls == kWedge) return kPivot;
+  if (triangle_schedule_ok(p.job)) return kRank;
+  if (edgepass_schedule_ok(p.job)) return kEdgePass;
+  return kPersistent;
+}
+
+Tune tune_for(fsgc_graph* g, const fsgc_tune* t, int sched)
+{
+  Tune r;
+  if (g && sched >= 0 && sched < kNumSchedules) {
+    std::lock_guard<std::mutex> lk(g->tune_mu);
+    if (g->tuned[sched]) r = g->best[sched];
+  }
+  if (!t) return r;
+  if (t->block > 0) r.block = t->block;
+  if (t->grid > 0) r.grid = t->grid;
+  if (t->chunk > 0) r.chunk = t->chunk;
+  if (t->hub_degree > 0) r.hub_degree = t->hub_degree;
+  if (t->lds_cap > 0) r.lds_cap = t->lds_cap;
+  return r;
+}
+
+size_t scratch_for(const fsgc_graph* g, int sched, const Tune& t)
+{
+  switch (sched) {
+    case kPivot: return pivot_scratch_bytes(g->dg, t);
+    case kRank: return triangle_scratch_bytes(g->dg, t);
+    case kEdgePass: return edgepass_scratch_bytes(g->dg, t);
+    default: return perroot_scratch_bytes(g->dg, t);
+  }
+}
+
+int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune, std::vector<JobResult>* res,
+                 double* kernel_ms, void* stream, std::string* err, int sample)
+{
+  std::vector<JobDesc> jobs;
+  std::vector<const DeviceGraph*> graphs;
+  std::vector<DeviceGraph> mgs(ps.size());
+  std::vector<int> data_job(ps.size(), -1), aut_job(ps.size(), -1);
+  const bool fast = !(tune && tune->schedule == 1);
+  DevScratch msc((hipStream_t)stream);  // motif graphs of the automorphism runs
+  int rc = 0;
+  for (size_t i = 0; i < ps.size(); i++) {
+    Prepared& p = ps[i];
+    if (p.need_aut && !p.aut_cached && sample <= 1) {
+      if (p.aut.hn > g->dg.n || p.aut.hm > g->dg.m) continue;  // count is 0 regardless
+      rc = device_graph_upload_async(p.mgraph, &mgs[i], &msc, err);
+      if (rc) break;
+      JobDesc hj = p.job;
+      hj.smod = 1;
+      hj.srem = 0;
+      aut_job[i] = (int)jobs.size();
+      jobs.push_back(hj);
+      graphs.push_back(&mgs[i]);
+    }
+    if (!p.trivial && schedule_of(p, fast) == kPersistent) {
+      data_job[i] = (int)jobs.size();
+      jobs.push_back(p.job);
+      graphs.push_back(&g->dg);
+    }
+  }
+  std::vector<JobResult> jr;
+  double ms_total = 0, ms = 0;
+  if (!rc) {
+    Tune t = tune_for(g, tune, kPersistent);
+    t.sample = sample;
+    rc = device_run_jobs(jobs, graphs, t, stream, &jr, &ms, err);
+  }
+  ms_total += ms;
+  if (log_on() && !jobs.empty()) fprintf(stderr, "[fsgc] per-root launch: %zu jobs, %.1f ms\n", jobs.size(), ms);
+  // each fast-schedule launch takes up to 8 motifs that share one enumeration
+  std::vector<JobResult> sres(ps.size());
+  for (int sched : {kPivot, kRank, kEdgePass}) {
+    if (rc || !fast) break;
+    std::vector<std::vector<size_t>> groups;
+    for (size_t i = 0; i < ps.size(); i++) {
+      if (ps[i].trivial || schedule_of(ps[i], fast) != sched) continue;
+      // tailed 4-cliques with different tail counts go to separate launches
+      const int key = (ps[i].job.cls == kTriangle && ps[i].job.kind == kTriTailedClique) ? ps[i].job.p[3] : -1;
+      bool placed = false;
+      for (auto& gr : groups) {
+        const auto& j0 = ps[gr[0]].job;
+        const int k0 = (j0.cls == kTriangle && j0.kind == kTriTailedClique) ? j0.p[3] : -1;
+        if ((int)gr.size() < kMaxPivotJobs && (key < 0 || k0 < 0 || key == k0)) { gr.push_back(i); placed = true; break; }
+      }
+      if (!placed) groups.push_back({i});
+    }
+    Tune t = tune_for(g, tune, sched);
+    t.sample = sample;
+    for (auto& gr : groups) {
+      std::vector<JobDesc> wj;
+      for (size_t k : gr) wj.push_back(ps[k].job);
+      std::vector<JobResult> r;
+      if (sched == kPivot) rc = device_run_wedge_pivot(wj, &g->dg, t, stream, &r, &ms, err);
+      else if (sched == kRank) rc = device_run_triangles(wj, &g->dg, t, stream, &r, &ms, err);
+      else rc = device_run_edgepass(wj, &g->dg, t, stream, &r, &ms, err);
+      if (rc) break;
+      ms_total += ms;
+      if (log_on()) fprintf(stderr, "[fsgc] %s launch: %zu motifs, %.1f ms\n", schedule_name(sched), gr.size(), ms);
+      for (size_t k = 0; k < r.size(); k++) sres[gr[k]] = r[k];
+    }
+  }
+  if (kernel_ms) *kernel_ms = ms_total;
+  if (rc) return rc;
+  res->assign(ps.size(), JobResult());
+  for (size_t i = 0; i < ps.size(); i++) {
+    if (data_job[i] >= 0) (*res)[i] = jr[data_job[i]];
+    else if (!ps[i].trivial) (*res)[i] = sres[i];
+    if (aut_job[i] >= 0) {
+      Prepared& p = ps[i];
+      p.aut.sum = jr[aut_job[i]].sum;
+      p.aut.flags = jr[aut_job[i]].flags;
+      p.aut_cached = true;
+      std::lock_guard<std::mutex> lk(g_aut_mu);
+      g_aut[p.key] = p.aut;
+    }
+  }
+  return 0;
+}
+
+}  // namespace fsgc

@@ -1,0 +1,69 @@
+// Host side of one counting call: motif preparation, kernel scheduling per class,
+// automorphism sums and the final division (shared by the C ABI, the autotuner and
+// the serving loop).
+#pragma once
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fsgc.h"
+#include "fsgc_plan.h"
+
+namespace fsgc {
+
+// Which kernel counts a motif on the data graph.
+enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2, kEdgePass = 3, kNumSchedules = 4 };
+const char* schedule_name(int s);
+
+// Automorphism sum of a motif (run on the motif graph, cached by the live fields).
+struct AutEntry {
+  u128 sum;
+  uint32_t flags;
+  int32_t hn;
+  int64_t hm;
+};
+
+struct Prepared {
+  MatchingOrder mo;
+  JobDesc job;
+  int qf = 1;
+  bool trivial = false;  // vertex count / edge count: no kernel
+  bool need_aut = false;
+  bool aut_cached = false;
+  AutEntry aut{};
+  std::string key;
+  HostGraph mgraph;
+};
+
+}  // namespace fsgc
+
+// A graph resident on the GPU, with the launch shapes the autotuner picked for it.
+struct fsgc_graph {
+  fsgc::DeviceGraph dg;
+  std::mutex tune_mu;
+  bool tuned[fsgc::kNumSchedules] = {false, false, false, false};
+  fsgc::Tune best[fsgc::kNumSchedules];
+};
+
+namespace fsgc {
+
+int ensure_init(std::string* err);  // selects the library's GPU on first use
+int current_device();
+void set_error(const std::string& msg);  // text returned by fsgc_last_error on this thread
+bool log_on();
+int prepare(const uint8_t* mob, const fsgc_tune* tune, Prepared* p, std::string* err);
+Schedule schedule_of(const Prepared& p, bool fast);
+// Launch shape for one schedule: the caller's non-zero fields, else the graph's tuned
+// values, else the defaults.
+Tune tune_for(fsgc_graph* g, const fsgc_tune* t, int sched);
+// Device scratch (bytes) one launch of `sched` takes on g with the given shape.
+size_t scratch_for(const fsgc_graph* g, int sched, const Tune& t);
+
+// Runs the data-graph jobs of all prepared motifs (and the motif-graph jobs of
+// motifs whose automorphism sum is not cached) on `stream`.
+// `sample` > 1 runs only every sample-th work item (timing runs of the autotuner).
+int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune, std::vector<JobResult>* res,
+                 double* kernel_ms, void* stream, std::string* err, int sample = 1);
+void finish(const Prepared& p, const JobResult& r, int32_t n, int64_t m, fsgc_result* out);
+
+}  // namespace fsgc

@@ -23,7 +23,18 @@ struct Tune {
   int chunk = 256;        // roots (CSR positions or vertices) per work item
   int hub_degree = 4096;  // roots of degree above this are split into single-root items
   int lds_cap = 2048;     // longest list staged in LDS by the wedge/triangle paths
+  int sample = 1;         // > 1: run only every sample-th work item (timing runs; partial counts)
 };
+
+// keeps every t.sample-th element of a work-item list (all of them when sample <= 1)
+template <class T>
+void sample_items(std::vector<T>* items, int sample)
+{
+  if (sample <= 1) return;
+  size_t k = 0;
+  for (size_t i = 0; i < items->size(); i += (size_t)sample) (*items)[k++] = (*items)[i];
+  items->resize(k);
+}
 
 // A graph resident in device memory (owned by the library).
 struct DeviceGraph {
@@ -41,7 +52,10 @@ struct DeviceGraph {
   long long tl_total = 0;
   long long* piv_cost = nullptr;         // n: sum of neighbour degrees
   std::vector<long long> h_piv_cost;
+  size_t derived_bytes = 0;              // device bytes of the derived arrays
 };
+
+struct DevScratch;
 
 struct JobResult {
   u128 sum = 0;
@@ -55,6 +69,7 @@ struct JobResult {
 int device_derived_build(DeviceGraph* g, void* stream, double* ms, std::string* err);
 void device_derived_free(DeviceGraph* g);
 constexpr int kMaxPivotJobs = 8;
+size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t);
 int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                            std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
@@ -62,6 +77,8 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
 constexpr int kMaxTriJobs = 8;
 bool triangle_schedule_ok(const JobDesc& j);
 bool edgepass_schedule_ok(const JobDesc& j);
+size_t triangle_scratch_bytes(const DeviceGraph& g, const Tune& t);
+size_t edgepass_scratch_bytes(const DeviceGraph& g, const Tune& t);
 int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                         std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
@@ -70,9 +87,11 @@ int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune&
 // kernels.hip
 int device_init(int device, std::string* err);
 int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err);
+int device_graph_upload_async(const HostGraph& g, DeviceGraph* dg, DevScratch* sc, std::string* err);
 void device_graph_free(DeviceGraph* dg);
 // Runs the given jobs (each bound to a graph) in one persistent launch on `stream`
 // and returns one result per job. kernel_ms receives the device time of the launch.
+size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t);
 int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
                     void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 

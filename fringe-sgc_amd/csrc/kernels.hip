@@ -15,6 +15,7 @@
 #include <cstring>
 
 #include "devmath.h"
+#include "devscratch.h"
 #include "fsgc_plan.h"
 
 namespace fsgc {
@@ -505,6 +506,12 @@ int device_init(int device, std::string* err)
     if (err) *err = "no such GPU";
     return -19;
   }
+  // keep freed call temporaries in the device's default pool for the next call
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   return 0;
 }
 
@@ -518,6 +525,23 @@ int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err)
   HIP_TRY(hipMalloc((void**)&dg->adj, sizeof(int32_t) * std::max<size_t>(1, (size_t)g.m)));
   HIP_TRY(hipMemcpy(dg->idx, g.nindex.data(), sizeof(int32_t) * ((size_t)g.n + 1), hipMemcpyHostToDevice));
   if (g.m) HIP_TRY(hipMemcpy(dg->adj, g.nlist.data(), sizeof(int32_t) * (size_t)g.m, hipMemcpyHostToDevice));
+  dg->h_idx = g.nindex;
+  return 0;
+}
+
+// A small graph (a motif graph for the automorphism run) in call temporaries on `stream`.
+int device_graph_upload_async(const HostGraph& g, DeviceGraph* dg, DevScratch* sc, std::string* err)
+{
+  dg->n = g.n;
+  dg->m = g.m;
+  dg->maxdeg = 0;
+  for (int32_t u = 0; u < g.n; u++) dg->maxdeg = std::max(dg->maxdeg, g.nindex[u + 1] - g.nindex[u]);
+  HIP_TRY(sc->get(&dg->idx, sizeof(int32_t) * ((size_t)g.n + 1)));
+  HIP_TRY(sc->get(&dg->adj, sizeof(int32_t) * std::max<size_t>(1, (size_t)g.m)));
+  HIP_TRY(hipMemcpyAsync(dg->idx, g.nindex.data(), sizeof(int32_t) * ((size_t)g.n + 1), hipMemcpyHostToDevice,
+                         sc->stream));
+  if (g.m)
+    HIP_TRY(hipMemcpyAsync(dg->adj, g.nlist.data(), sizeof(int32_t) * (size_t)g.m, hipMemcpyHostToDevice, sc->stream));
   dg->h_idx = g.nindex;
   return 0;
 }
@@ -569,6 +593,13 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
   emit();
 }
 
+size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t)
+{
+  // work items: at most one per `chunk` CSR positions plus one per split hub piece
+  const size_t per = (size_t)std::max(1, std::min(t.chunk, 64));
+  return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 64) + 4096;
+}
+
 int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
                     void* stream_v, std::vector<JobResult>* out, double* kernel_ms, std::string* err)
 {
@@ -585,17 +616,19 @@ int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGr
     jobs[k].m = (int32_t)graphs[k]->m;
     cut_items(jobs[k], *graphs[k], k, t, &items);
   }
+  sample_items(&items, t.sample);
   // most expensive items first: order by job, big graphs first
+  DevScratch sc(stream);
   Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
   int* d_head = nullptr;
   const size_t acc_bytes = sizeof(Accum) * nj;
-  HIP_TRY(hipMalloc((void**)&d_acc, acc_bytes + 256));
+  HIP_TRY(sc.get(&d_acc, acc_bytes + 256));
   d_head = (int*)((char*)d_acc + acc_bytes + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(hipMalloc((void**)&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(sc.get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
   HIP_TRY(hipMemsetAsync(d_acc, 0, acc_bytes + 256, stream));
   HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
   if (!items.empty())
@@ -606,26 +639,20 @@ int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGr
   int grid = t.grid > 0 ? t.grid : cus * 8;
   const int nitems = (int)items.size();
   grid = std::max(1, std::min(grid, (nitems + dev::kWavesPerBlock - 1) / dev::kWavesPerBlock));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  EventPair ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.a, stream));
   if (nitems > 0) {
     hipLaunchKernelGGL(dev::k_count, dim3(grid), dim3(dev::kBlock), 0, stream, d_jobs, d_items, nitems, d_head);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipEventRecord(ev.b, stream));
   std::vector<Accum> h_acc(nj);
   HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, acc_bytes, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (kernel_ms) *kernel_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(d_items);
-  (void)hipFree(d_jobs);
-  (void)hipFree(d_acc);
   for (int k = 0; k < nj; k++) {
     JobResult& r = (*out)[k];
     r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;

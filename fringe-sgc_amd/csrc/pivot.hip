@@ -21,9 +21,11 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "devmath.h"
+#include "devscratch.h"
 #include "fsgc_plan.h"
 
 namespace fsgc {
@@ -390,19 +392,22 @@ void device_derived_free(DeviceGraph* g)
 // Builds the per-edge supports and common-neighbour lists (once per graph).
 int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string* err)
 {
+  // one build per graph, also when several host threads count on it at once
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
   if (g->derived_ready) { if (ms) *ms = 0; return 0; }
   hipStream_t stream = (hipStream_t)stream_v;
   const int n = g->n;
   const long long m = g->m;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  EventPair ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.a, stream));
   HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
   HIP_TRY(hipMalloc((void**)&g->tl_off, sizeof(long long) * (m + 1)));
   HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
+  DevScratch sc(stream);
   int* d_head = nullptr;
-  HIP_TRY(hipMalloc((void**)&d_head, sizeof(int) * 2));
+  HIP_TRY(sc.get(&d_head, sizeof(int) * 2));
   HIP_TRY(hipMemsetAsync(d_head, 0, sizeof(int) * 2, stream));
   int dev = 0, cus = 256;
   HIP_TRY(hipGetDevice(&dev));
@@ -417,12 +422,11 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   hipcub::TransformInputIterator<long long, dev::Widen, const int*> in(g->tsup, dev::Widen());
   HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
   void* tmp = nullptr;
-  HIP_TRY(hipMalloc(&tmp, tmp_bytes + 16));
+  HIP_TRY(sc.get(&tmp, tmp_bytes + 16));
   HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
   long long total = 0;
   HIP_TRY(hipMemcpyAsync(&total, g->tl_off + m, sizeof(long long), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
-  (void)hipFree(tmp);
   g->tl_total = total;
   HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
   HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
@@ -431,18 +435,35 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipEventRecord(ev.b, stream));
+  g->h_piv_cost.resize(n);
+  HIP_TRY(hipMemcpyAsync(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float t = 0;
-  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  HIP_TRY(hipEventElapsedTime(&t, ev.a, ev.b));
   if (ms) *ms = t;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(d_head);
-  g->h_piv_cost.resize(n);
-  HIP_TRY(hipMemcpy(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost));
+  g->derived_bytes = sizeof(int) * (size_t)m + sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total +
+                     sizeof(long long) * (size_t)n;
   g->derived_ready = true;
   return 0;
+}
+
+// Grid of the pivot launch (before clamping to the number of items) and the device
+// scratch it takes: one dense D array of n counters per workgroup and one spill array
+// of maxdeg counters per wave for centres whose list does not fit in LDS.
+static int pivot_grid(const Tune& t)
+{
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return t.grid > 0 ? t.grid : cus * 4;
+}
+
+size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
+{
+  const size_t grid = (size_t)pivot_grid(t);
+  const size_t dstride = ((size_t)g.n + 63) / 64 * 64, estride = ((size_t)g.maxdeg + 63) / 64 * 64;
+  return sizeof(unsigned) * grid * (dstride + estride * dev::kPivWaves) + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
 }
 
 // Runs up to kMaxPivotJobs wedge-class jobs on one graph in one pivot-schedule launch.
@@ -483,34 +504,31 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
   std::vector<dev::PivotItem> sorted(items.size());
   for (size_t i = 0; i < order.size(); i++) sorted[i] = items[order[i]];
+  sample_items(&sorted, t.sample);
 
-  int dev = 0, cus = 256;
-  HIP_TRY(hipGetDevice(&dev));
-  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int grid = t.grid > 0 ? t.grid : cus * 4;
-  grid = std::max(1, std::min<int>(grid, (int)sorted.size()));
+  const int grid = std::max(1, std::min<int>(pivot_grid(t), (int)sorted.size()));
   const long long dstride = ((long long)g->n + 63) / 64 * 64;
   const long long estride = ((long long)g->maxdeg + 63) / 64 * 64;
+  DevScratch sc(stream);
   unsigned* dbuf = nullptr;
   unsigned* ebuf = nullptr;
   Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   dev::PivotItem* d_items = nullptr;
-  HIP_TRY(hipMalloc((void**)&dbuf, sizeof(unsigned) * dstride * grid));
-  HIP_TRY(hipMalloc((void**)&ebuf, sizeof(unsigned) * estride * grid * dev::kPivWaves));
-  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
+  HIP_TRY(sc.get(&ebuf, sizeof(unsigned) * estride * grid * dev::kPivWaves));
+  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
   int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(hipMalloc((void**)&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
+  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(sc.get(&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
   HIP_TRY(hipMemsetAsync(dbuf, 0, sizeof(unsigned) * dstride * grid, stream));
   HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
   HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice, stream));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  EventPair ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.a, stream));
   const int nitems = (int)sorted.size();
   const int c3 = need_c3 ? 1 : 0;
   bool full = false;  // any fringe vector with a multi-vertex anchor set
@@ -530,20 +548,13 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
 #undef FSGC_PIVOT_LAUNCH
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipEventRecord(ev.b, stream));
   std::vector<Accum> h_acc(nj);
   HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (kernel_ms) *kernel_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(d_items);
-  (void)hipFree(d_jobs);
-  (void)hipFree(d_acc);
-  (void)hipFree(ebuf);
-  (void)hipFree(dbuf);
   for (int k = 0; k < nj; k++) {
     JobResult& r = (*out)[k];
     r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;

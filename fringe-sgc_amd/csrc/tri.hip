@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "devmath.h"
+#include "devscratch.h"
 #include "fsgc_plan.h"
 
 namespace fsgc {
@@ -277,6 +278,28 @@ bool edgepass_schedule_ok(const JobDesc& j)
   return (j.cls == kEdge && j.kind == kEdgeGeneral) || (j.cls == kVertex && j.kind == kVertexStars);
 }
 
+// Grid of the rank launch (before clamping to the items) and its device scratch: one
+// byte-mark array of maxdeg entries per wave for vertices whose list exceeds LDS.
+static int triangle_grid(const Tune& t)
+{
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return t.grid > 0 ? t.grid : cus * 4;
+}
+
+size_t triangle_scratch_bytes(const DeviceGraph& g, const Tune& t)
+{
+  const size_t mstride = ((size_t)g.maxdeg + 63) / 64 * 64;
+  return mstride * (size_t)triangle_grid(t) * dev::kTriWaves + sizeof(dev::TriItem) * ((size_t)g.n + 1024);
+}
+
+size_t edgepass_scratch_bytes(const DeviceGraph& g, const Tune& t)
+{
+  const size_t step = (size_t)std::max<long long>(256, (long long)t.chunk * 32);
+  return sizeof(WorkItem) * ((size_t)g.m / step + 2) + 4096;
+}
+
 // Runs up to kMaxTriJobs closed-form edge and star jobs in one sweep over the edges.
 int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
                         std::vector<JobResult>* out, double* kernel_ms, std::string* err)
@@ -300,26 +323,27 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
     const int u1 = (int)(std::upper_bound(hi.begin(), hi.end(), (int32_t)(e - 1)) - hi.begin()) - 1;
     items.push_back(WorkItem{0, u0, u1, 0, b, e});
   }
+  sample_items(&items, t.sample);
   int dev = 0, cus = 256;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int grid = t.grid > 0 ? t.grid : cus * 8;
   grid = std::max(1, std::min<int>(grid, ((int)items.size() + 3) / 4));
+  DevScratch sc(stream);
   Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
-  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
   int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(hipMalloc((void**)&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(sc.get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
   HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
   HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, stream));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  EventPair ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.a, stream));
   if (!items.empty()) {
     if (nj == 1)
       hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, stream, g->idx, g->adj, g->tsup, d_jobs, nj,
@@ -329,18 +353,13 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
                          g->tsup, d_jobs, nj, d_items, (int)items.size(), d_head);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipEventRecord(ev.b, stream));
   std::vector<Accum> h_acc(nj);
   HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (kernel_ms) *kernel_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(d_items);
-  (void)hipFree(d_jobs);
-  (void)hipFree(d_acc);
   for (int k = 0; k < nj; k++) {
     JobResult& r = (*out)[k];
     r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
@@ -389,30 +408,28 @@ int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune&
     const int per = (int)std::max<long long>(1, budget * 64 / std::max(64, dx));
     for (int p = 0; p < dx; p += per) items.push_back(dev::TriItem{x, p, std::min(dx, p + per), 0});
   }
-  int dev = 0, cus = 256;
-  HIP_TRY(hipGetDevice(&dev));
-  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int grid = t.grid > 0 ? t.grid : cus * 4;
+  sample_items(&items, t.sample);
+  int grid = triangle_grid(t);
   grid = std::max(1, std::min<int>(grid, ((int)items.size() + dev::kTriWaves - 1) / dev::kTriWaves));
   const long long mstride = ((long long)g->maxdeg + 63) / 64 * 64;
+  DevScratch sc(stream);
   unsigned char* gmark = nullptr;
   Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   dev::TriItem* d_items = nullptr;
-  HIP_TRY(hipMalloc((void**)&gmark, (size_t)mstride * grid * dev::kTriWaves));
-  HIP_TRY(hipMalloc((void**)&d_acc, sizeof(Accum) * nj + 256));
+  HIP_TRY(sc.get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
+  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
   int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(hipMalloc((void**)&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(hipMalloc((void**)&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
+  HIP_TRY(sc.get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
   HIP_TRY(hipMemsetAsync(gmark, 0, (size_t)mstride * grid * dev::kTriWaves, stream));
   HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
   HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice, stream));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
+  EventPair ev;
+  HIP_TRY(ev.create());
+  HIP_TRY(hipEventRecord(ev.a, stream));
   const int nitems = (int)items.size();
   if (nitems > 0) {
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
@@ -428,19 +445,13 @@ int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune&
 #undef FSGC_TRI_LAUNCH
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(e1, stream));
+  HIP_TRY(hipEventRecord(ev.b, stream));
   std::vector<Accum> h_acc(nj);
   HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (kernel_ms) *kernel_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(d_items);
-  (void)hipFree(d_jobs);
-  (void)hipFree(d_acc);
-  (void)hipFree(gmark);
   for (int k = 0; k < nj; k++) {
     JobResult& r = (*out)[k];
     r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;

@@ -41,6 +41,31 @@ class Tune(C.Structure):
     ]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("schedule", C.c_int32), ("candidates", C.c_int32), ("best", Tune),
+                ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch", C.c_int32), ("scratch_cap", C.c_uint64),
+                ("grid", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _Query(C.Structure):
+    _fields_ = [("result", _Result), ("status", C.c_int32), ("graph", C.c_int32), ("batch_size", C.c_int32),
+                ("slot", C.c_int32), ("submit_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
+                ("scratch_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+                ("rejected", C.c_uint64), ("scratch_cap", C.c_uint64), ("scratch_peak", C.c_uint64),
+                ("slots", C.c_int32), ("peak_concurrent", C.c_int32)]
+
+
+SCHEDULE_NAMES = ["per-root", "pivot", "rank", "edge-pass"]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libfsgc.so not built at {LIB_PATH}; run `make -C fringe-sgc_amd`")
@@ -64,6 +89,16 @@ def _load():
         "fsgc_free_host": (None, [C.c_void_p]),
         "fsgc_random_motifs": (C.c_int, [C.c_uint64, C.c_int, C.c_char_p]),
         "fsgc_random_patterns": (C.c_int, [C.c_uint64, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "fsgc_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int32, P(TuneReport), P(C.c_int32)]),
+        "fsgc_graph_untune": (C.c_int, [C.c_void_p]),
+        "fsgc_server_create": (C.c_int, [P(ServerConfig), P(C.c_void_p)]),
+        "fsgc_server_add_graph": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_int32)]),
+        "fsgc_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, P(C.c_uint64)]),
+        "fsgc_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, P(_Query)]),
+        "fsgc_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(_Query)]),
+        "fsgc_server_stats_get": (C.c_int, [C.c_void_p, P(ServerStats)]),
+        "fsgc_server_clock_ms": (C.c_double, [C.c_void_p]),
+        "fsgc_server_destroy": (None, [C.c_void_p]),
         "fsgc_last_error": (C.c_char_p, []),
         "fsgc_strerror": (C.c_char_p, [C.c_int]),
     }
@@ -169,6 +204,23 @@ class Graph:
         _check(lib.fsgc_count_batch(self._h, buf, k, C.byref(tune) if tune else None, out, C.byref(ms)))
         return [_result(out[i]) for i in range(k)], ms.value
 
+    def autotune(self, mos: list[bytes], sample: int = 16) -> list[dict]:
+        """Times launch shapes per schedule on every sample-th work item and keeps the
+        fastest for this graph (used by later counts for fields left at zero)."""
+        buf = b"".join(mos)
+        reps = (TuneReport * 4)()
+        nr = C.c_int32()
+        _check(lib.fsgc_autotune(self._h, buf, len(mos), sample, reps, C.byref(nr)))
+        out = []
+        for r in reps[: nr.value]:
+            out.append({"schedule": SCHEDULE_NAMES[r.schedule], "candidates": r.candidates,
+                        "grid": r.best.grid, "chunk": r.best.chunk, "hub_degree": r.best.hub_degree,
+                        "best_ms": r.best_ms, "default_ms": r.default_ms})
+        return out
+
+    def untune(self):
+        _check(lib.fsgc_graph_untune(self._h))
+
     def free(self):
         if self._h:
             lib.fsgc_graph_free(self._h)
@@ -243,3 +295,83 @@ def live_fields(mo: bytes) -> dict:
         "destdeg": list(mo[3:3 + max(0, nodes - 1)]), "list": list(mo[8:8 + max(0, nodes - 1)]),
         "anch": list(mo[13:13 + nas]), "fcnt": list(mo[76:76 + nas]),
     }
+
+
+@dataclass
+class QueryResult:
+    result: Result
+    status: int
+    graph: int
+    batch_size: int
+    slot: int
+    submit_ms: float
+    start_ms: float
+    done_ms: float
+    scratch_bytes: int
+
+    @property
+    def latency_ms(self) -> float:
+        return self.done_ms - self.submit_ms
+
+
+def _query(q: _Query) -> QueryResult:
+    return QueryResult(result=_result(q.result), status=q.status, graph=q.graph, batch_size=q.batch_size,
+                       slot=q.slot, submit_ms=q.submit_ms, start_ms=q.start_ms, done_ms=q.done_ms,
+                       scratch_bytes=q.scratch_bytes)
+
+
+class Server:
+    """The library's serving loop: resident graphs, queries folded into shared
+    launches on `slots` concurrent streams, admitted by device scratch."""
+
+    def __init__(self, slots: int = 2, max_batch: int = 8, scratch_cap: int = 0, grid: int = 0):
+        cfg = ServerConfig(slots=slots, max_batch=max_batch, scratch_cap=scratch_cap, grid=grid)
+        h = C.c_void_p()
+        _check(lib.fsgc_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._graphs = []
+
+    def add_graph(self, g: Graph) -> int:
+        gid = C.c_int32()
+        _check(lib.fsgc_server_add_graph(self._h, g._h, C.byref(gid)))
+        self._graphs.append(g)  # keep it alive as long as the server
+        return gid.value
+
+    def submit(self, graph: int, mo: bytes) -> int:
+        assert len(mo) == MO_BYTES
+        t = C.c_uint64()
+        _check(lib.fsgc_server_submit(self._h, graph, mo, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> QueryResult:
+        q = _Query()
+        _check(lib.fsgc_server_wait(self._h, ticket, C.byref(q)))
+        return _query(q)
+
+    def poll(self, ticket: int):
+        q = _Query()
+        rc = lib.fsgc_server_poll(self._h, ticket, C.byref(q))
+        if rc == 0:
+            return None
+        if rc < 0:
+            _check(rc)
+        return _query(q)
+
+    def clock_ms(self) -> float:
+        return lib.fsgc_server_clock_ms(self._h)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib.fsgc_server_stats_get(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib.fsgc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,6 +16,10 @@
  *   fsgc_preprocess      fringePreprocess n s d s d ...        src/fringePreprocess.cpp:287-365
  *   fsgc_preprocess_pattern  fringePreprocess <pattern#>       src/fringePreprocess.cpp:296-298, src/patterns.h:57-354
  *   fsgc_print_matching_order  printMatchingOrder              src/fringes.h:61-98
+ *   fsgc_autotune        (no counterpart: the reference's launch shape is fixed,
+ *                         512 threads per block, fringeCount.cu:82-84, 2276-2282)
+ *   fsgc_server_*        (no counterpart: the reference is one motif per process,
+ *                         driven by reproduce.py:69-73 through subprocess)
  */
 #ifndef FSGC_H
 #define FSGC_H
@@ -59,11 +63,13 @@ typedef struct fsgc_result {
   double index_ms;             /* device time spent building the graph's per-edge index in this call (0 if cached) */
 } fsgc_result;
 
+/* Launch shape. A zero field takes the value fsgc_autotune picked for the graph and
+   the motif's schedule, or the default when the graph was not tuned. */
 typedef struct fsgc_tune {
   int32_t block;       /* threads per workgroup; 0 = default (256) */
-  int32_t grid;        /* persistent workgroups; 0 = 8 per CU */
-  int32_t chunk;       /* roots per work item; 0 = default */
-  int32_t hub_degree;  /* roots above this degree get items of their own; 0 = default */
+  int32_t grid;        /* persistent workgroups; 0 = tuned or default (4-8 per CU) */
+  int32_t chunk;       /* roots per work item; 0 = tuned or default */
+  int32_t hub_degree;  /* roots above this degree get items of their own; 0 = tuned or default */
   int32_t lds_cap;     /* longest neighbour list staged in LDS; 0 = default */
   int32_t slice_mod;   /* > 1: count only roots u with u % slice_mod == slice_rem */
   int32_t slice_rem;
@@ -89,6 +95,69 @@ int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* 
    fsgc_count of mos + i*139 alone. batch_ms receives the device time of the launch. */
 int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* tune, fsgc_result* out,
                      double* batch_ms);
+
+/* Autotuner. For every schedule the given motifs use (per-root kernel, pivot, rank,
+   edge pass), times candidate launch shapes (workgroups, roots per work item, hub
+   threshold) on every sample-th work item of the graph (sample <= 1: 16) and keeps the
+   fastest for the graph; later counts on the graph use it for fields left at zero.
+   Counts never depend on the shape. reports[0..*nreports) describe each schedule
+   tuned (up to FSGC_NUM_SCHEDULES). */
+#define FSGC_NUM_SCHEDULES 4
+typedef struct fsgc_tune_report {
+  int32_t schedule;     /* 0 per-root, 1 pivot, 2 rank, 3 edge pass */
+  int32_t candidates;   /* shapes timed */
+  fsgc_tune best;       /* the shape kept */
+  double best_ms;       /* device time of the sample with it */
+  double default_ms;    /* device time of the sample with the default shape */
+} fsgc_tune_report;
+int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc_tune_report* reports,
+                  int32_t* nreports);
+/* forget the tuned shapes of a graph */
+int fsgc_graph_untune(fsgc_graph* g);
+
+/* Serving loop. A server keeps registered graphs resident (with their per-edge
+   index), takes count queries at any time from any thread, and runs them on `slots`
+   concurrent HIP streams. Whenever a slot is free it folds the waiting queries that
+   share a graph and a schedule (up to max_batch, oldest first) into one launch,
+   admitted only while the device scratch of the launches in flight stays within
+   scratch_cap; queries arriving while others run join the next launch on a free
+   slot. Each result is identical to fsgc_count of the same motif alone. */
+typedef struct fsgc_server fsgc_server;
+typedef struct fsgc_server_config {
+  int32_t slots;         /* concurrent launches (streams); 0 = 2 */
+  int32_t max_batch;     /* motifs folded into one launch, 1..8; 0 = 8 */
+  uint64_t scratch_cap;  /* device bytes the launches in flight may hold; 0 = half the free memory */
+  int32_t grid;          /* workgroups per launch; 0 = the machine's share of one slot */
+  int32_t reserved;
+} fsgc_server_config;
+typedef struct fsgc_query {
+  fsgc_result result;
+  int32_t status;        /* 0, or the negative error code of the query */
+  int32_t graph;         /* graph id the query ran on */
+  int32_t batch_size;    /* motifs in the launch that counted it */
+  int32_t slot;          /* stream slot */
+  double submit_ms;      /* server clock (fsgc_server_clock_ms) at submission */
+  double start_ms;       /* ... when its launch was admitted */
+  double done_ms;        /* ... when its result was ready */
+  uint64_t scratch_bytes;/* device scratch of its launch */
+} fsgc_query;
+typedef struct fsgc_server_stats {
+  uint64_t submitted, completed, launches, rejected;
+  uint64_t scratch_cap, scratch_peak; /* bytes */
+  int32_t slots, peak_concurrent;     /* launches in flight at once (max seen) */
+} fsgc_server_stats;
+int fsgc_server_create(const fsgc_server_config* cfg, fsgc_server** out);
+/* registers a graph (builds its per-edge index now); g must outlive the server */
+int fsgc_server_add_graph(fsgc_server* s, fsgc_graph* g, int32_t* id);
+int fsgc_server_submit(fsgc_server* s, int32_t graph, const uint8_t mo[FSGC_MO_BYTES], uint64_t* ticket);
+/* blocks until the query is done, fills out and forgets the ticket */
+int fsgc_server_wait(fsgc_server* s, uint64_t ticket, fsgc_query* out);
+/* 1 and out filled (ticket forgotten) if done, 0 if still pending */
+int fsgc_server_poll(fsgc_server* s, uint64_t ticket, fsgc_query* out);
+int fsgc_server_stats_get(fsgc_server* s, fsgc_server_stats* out);
+double fsgc_server_clock_ms(const fsgc_server* s);
+/* finishes the queries already submitted, then stops */
+void fsgc_server_destroy(fsgc_server* s);
 
 /* Motif compiler: pattern with `nodes` vertices and `nedges` edges (pairs in `edges`)
    -> 139-byte matching order (unused slots zero). `log` receives the reference's
