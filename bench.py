@@ -72,44 +72,173 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def batch_compare(spec, seed=3, count=64, max_core=3, out_path=None):
+def batch_compare(spec, seed=3, count=64, classes=None, out_path=None):
     """Plain path (one fsgc_count per motif) against the batched path (one
-    fsgc_count_batch) on the same resident graph: motifs/s of each, and per-motif
-    equality of every result. Motifs with cores of 4-6 vertices are left out when
-    max_core < 4 (their reference enumeration is exponential in the core size)."""
+    fsgc_count_batch over all motifs) on the same resident graph: motifs/s of each,
+    per-class times, and per-motif equality of every result. `classes` restricts the
+    motif set to the named classes (the others are listed as excluded)."""
     import fsgc
     fsgc.init(0)
+    t0 = time.perf_counter()
     g = fsgc.Graph.generate(spec)
+    n, m, maxdeg = g.info()
+    print(f"# graph {spec}: n={n} m={m} maxdeg={maxdeg} ({time.perf_counter() - t0:.1f} s)", flush=True)
     mos_all = fsgc.random_motifs(seed, count)
-    mos = [m for m in mos_all if m[0] <= max_core]
-    # automorphism sums and the per-edge index are per motif / per graph: warm them once
+    cls_of = {i: fsgc.motif_class(mo) for i, mo in enumerate(mos_all)}
+    keep = [i for i in range(count) if classes is None or cls_of[i] in classes]
+    mos = [mos_all[i] for i in keep]
+    # automorphism sums are per motif and the per-edge index per graph: warm them once
     t0 = time.perf_counter()
     g.count_batch(mos)
     print(f"# warm batch (index + automorphism sums) {time.perf_counter() - t0:.1f} s", flush=True)
-    plain, t_plain = [], 0.0
-    for i, mo in enumerate(mos):
+    plain, t_plain, per_class = [], 0.0, {}
+    for i, mo in zip(keep, mos):
         t0 = time.perf_counter()
         r = g.count(mo)
         dt = time.perf_counter() - t0
         t_plain += dt
         plain.append(r)
-        print(f"# plain motif {i} ({r.class_name}): {dt:.2f} s", flush=True)
+        pc = per_class.setdefault(r.class_name, {"motifs": 0, "plain_s": 0.0})
+        pc["motifs"] += 1
+        pc["plain_s"] += dt
+        print(f"# plain motif {i} ({r.class_name}): {dt:.3f} s", flush=True)
     t0 = time.perf_counter()
     batch, batch_ms = g.count_batch(mos)
     t_batch = time.perf_counter() - t0
-    same = all((a.exact, a.sg, a.aut, a.ref_compat, a.core_matches) == (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches)
-               for a, b in zip(plain, batch))
+    print(f"# batched: {t_batch:.3f} s", flush=True)
+    same = all((a.exact, a.sg, a.aut, a.ref_compat, a.core_matches, a.flags) ==
+               (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches, b.flags) for a, b in zip(plain, batch))
+    # the same batch captured once as a HIP graph, then replayed
+    cb = g.capture_batch(mos)
+    t0 = time.perf_counter()
+    cap, cap_ms = cb.run()
+    t_cap = time.perf_counter() - t0
+    cb.free()
+    same_cap = all((a.exact, a.sg, a.core_matches) == (b.exact, b.sg, b.core_matches) for a, b in zip(plain, cap))
+    print(f"# captured batch replay: {t_cap:.3f} s, identical {same_cap}", flush=True)
+    # the same per class (one count_batch per class)
+    for c, pc in per_class.items():
+        sub = [mo for mo, r in zip(mos, plain) if r.class_name == c]
+        t0 = time.perf_counter()
+        g.count_batch(sub)
+        pc["batched_s"] = time.perf_counter() - t0
+        pc["speedup"] = round(pc["plain_s"] / pc["batched_s"], 3)
+        pc["plain_s"] = round(pc["plain_s"], 4)
+        pc["batched_s"] = round(pc["batched_s"], 4)
+        print(f"# class {c}: {pc}", flush=True)
+    excluded = {}
+    for i in range(count):
+        if i not in keep:
+            excluded[cls_of[i]] = excluded.get(cls_of[i], 0) + 1
     res = {
-        "graph": spec, "motifs": len(mos), "motifs_generated": count, "seed": seed,
-        "excluded": f"{count - len(mos)} motifs with 4-6 core vertices",
+        "graph": spec, "nodes": n, "directed_edges": m, "maxdeg": maxdeg,
+        "motifs": len(mos), "motifs_generated": count, "seed": seed, "excluded": excluded,
         "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
         "plain_motifs_per_s": round(len(mos) / t_plain, 4), "batched_motifs_per_s": round(len(mos) / t_batch, 4),
         "speedup": round(t_plain / t_batch, 3), "identical": same,
-        "classes": {n: sum(1 for r in plain if r.class_name == n) for n in fsgc.CLASS_NAMES},
+        "captured_s": round(t_cap, 3), "captured_speedup": round(t_plain / t_cap, 3), "captured_identical": same_cap,
+        "per_class": per_class,
     }
     print(json.dumps(res), flush=True)
     if out_path:
         json.dump({"summary": res, "counts": [str(r.exact) for r in batch]}, open(out_path, "w"), indent=1)
+    return res
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = max(0, min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def serve_bench(specs, rate, nq, slots, classes=None, seed=5, out_path=None, check=True):
+    """Served query stream: graphs resident, Poisson arrivals at `rate` queries/s from
+    `seed`, each query a uniform graph and the next motif of the random-motif generator
+    (seed `seed`), submitted to the library's server at its arrival time. Latency is
+    completion minus arrival on the server's clock. Every result is then recomputed
+    alone with fsgc_count and compared."""
+    import threading
+
+    import numpy as np
+
+    import fsgc
+    fsgc.init(0)
+    graphs = []
+    for spec in specs:
+        t0 = time.perf_counter()
+        g = fsgc.Graph.generate(spec)
+        n, m, maxdeg = g.info()
+        graphs.append(g)
+        print(f"# graph {spec}: n={n} m={m} maxdeg={maxdeg} ({time.perf_counter() - t0:.1f} s)", flush=True)
+    rng = np.random.default_rng(seed)
+    pool = fsgc.random_motifs(seed, 4 * nq)
+    queries = []
+    for mo in pool:
+        if len(queries) == nq:
+            break
+        c = fsgc.motif_class(mo)
+        if classes is not None and c not in classes:
+            continue
+        queries.append((int(rng.integers(len(graphs))), mo, c))
+    gaps = rng.exponential(1.0 / rate, size=len(queries))
+    srv = fsgc.Server(slots=slots)
+    t0 = time.perf_counter()
+    ids = [srv.add_graph(g) for g in graphs]
+    print(f"# server: {slots} slots, graphs registered (index build {time.perf_counter() - t0:.1f} s)", flush=True)
+    tickets = [None] * len(queries)
+
+    def submitter():
+        start = time.perf_counter()
+        at = 0.0
+        for i, (gi, mo, c) in enumerate(queries):
+            at += gaps[i]
+            delay = start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(ids[gi], mo)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    results = []
+    for i, t in enumerate(tickets):
+        q = srv.wait(t)
+        results.append(q)
+        print(f"# query {i} ({queries[i][2]} on graph {queries[i][0]}): latency {q.latency_ms:.1f} ms, "
+              f"batch {q.batch_size}, slot {q.slot}", flush=True)
+    st = srv.stats()
+    srv.close()
+    lat = [q.latency_ms for q in results]
+    first = min(q.submit_ms for q in results)
+    last = max(q.done_ms for q in results)
+    identical = None
+    if check:
+        identical = True
+        for (gi, mo, c), q in zip(queries, results):
+            a = graphs[gi].count(mo)
+            b = q.result
+            identical = identical and (a.exact, a.sg, a.aut, a.ref_compat, a.core_matches, a.flags) == \
+                (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches, b.flags)
+        print(f"# alone == served for every query: {identical}", flush=True)
+    by_class = {}
+    for (gi, mo, c), q in zip(queries, results):
+        by_class.setdefault(c, []).append(q.latency_ms)
+    res = {
+        "graphs": specs, "offered_rate_qps": rate, "queries": len(queries), "slots": slots, "seed": seed,
+        "classes": sorted(by_class), "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+        "mean_ms": round(sum(lat) / len(lat), 3), "max_ms": round(max(lat), 3),
+        "motifs_per_s": round(len(results) / ((last - first) / 1000.0), 3),
+        "launches": st["launches"], "peak_concurrent": st["peak_concurrent"],
+        "scratch_peak_bytes": st["scratch_peak"], "scratch_cap_bytes": st["scratch_cap"],
+        "mean_batch": round(sum(q.batch_size for q in results) / len(results), 3),
+        "per_class_p50_p99_ms": {c: [round(percentile(v, 50), 3), round(percentile(v, 99), 3)] for c, v in by_class.items()},
+        "identical_alone_vs_served": identical,
+    }
+    print(json.dumps(res), flush=True)
+    if out_path:
+        json.dump(res, open(out_path, "w"), indent=1)
     return res
 
 
@@ -120,12 +249,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="autotune the launch shape on the workload first")
+    ap.add_argument("--tune-sample", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
     ap.add_argument("--batch", default=None, help="graph spec: compare plain vs batched over the random motifs")
     ap.add_argument("--batch-out", default=None)
+    ap.add_argument("--classes", default=None, help="comma-separated motif classes for --batch / --serve")
+    ap.add_argument("--serve", default=None, help="comma-separated graph specs: served Poisson query stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="--serve: offered queries per second")
+    ap.add_argument("--queries", type=int, default=64, help="--serve: number of queries")
+    ap.add_argument("--slots", type=int, default=2, help="--serve: concurrent launches")
+    ap.add_argument("--serve-out", default=None)
     args = ap.parse_args()
+    classes = set(args.classes.split(",")) if args.classes else None
     if args.batch:
-        batch_compare(args.batch, out_path=args.batch_out)
+        batch_compare(args.batch, classes=classes, out_path=args.batch_out)
+        return
+    if args.serve:
+        serve_bench(args.serve.split(","), args.rate, args.queries, args.slots, classes=classes,
+                    out_path=args.serve_out)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -147,6 +289,11 @@ def main():
     first = g.count(mo)
     print(f"# first call: index {first.index_ms:.1f} ms, kernel {first.kernel_ms:.1f} ms, class {first.class_name}, "
           f"exact {first.exact}", flush=True)
+    tuned = None
+    if args.tune:
+        t0 = time.perf_counter()
+        tuned = g.autotune([mo], sample=args.tune_sample)
+        print(f"# autotune ({time.perf_counter() - t0:.1f} s): {tuned}", flush=True)
     for i in range(args.warmup):
         r = g.count(mo)
         print(f"# warmup {i}: kernel {r.kernel_ms:.1f} ms", flush=True)
@@ -234,6 +381,7 @@ def main():
             "ref_compat_u64": r.ref_compat,
             "flags": r.flags,
             "index_build_ms": round(first.index_ms, 3),
+            "autotuned": tuned,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

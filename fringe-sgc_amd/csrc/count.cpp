@@ -135,6 +135,74 @@ size_t scratch_for(const fsgc_graph* g, int sched, const Tune& t)
   }
 }
 
+namespace {
+
+// Motifs share a launch only within a tier: a launch runs the kernel variant its most
+// demanding motif needs (the c3 pass, the full fc3 loop), so mixing tiers would make
+// the cheap motifs pay for it.
+int group_tier(const JobDesc& j)
+{
+  bool multi = false;  // a fringe vector with a multi-vertex anchor set (full fc3)
+  for (int r = 0; r < j.rot; r++) multi = multi || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+  if (j.cls == kEdge) return 0;
+  if (j.cls == kWedge) return j.kind == kWedgeSpecial ? 0 : (multi ? 2 : 1);
+  if (j.cls == kTriangle) return j.kind != kTriGeneral ? 1 : (multi ? 3 : 2);
+  return 0;
+}
+
+}  // namespace
+
+std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast)
+{
+  std::vector<Group> out;
+  Group perroot{kPersistent, {}};
+  for (size_t i = 0; i < ps.size(); i++)
+    if (!ps[i].trivial && schedule_of(ps[i], fast) == kPersistent) perroot.idx.push_back(i);
+  if (!perroot.idx.empty()) out.push_back(perroot);
+  if (!fast) return out;
+  // each fast-schedule launch takes up to 8 motifs that share one enumeration
+  for (int sched : {kPivot, kRank, kEdgePass}) {
+    std::vector<Group> groups;
+    for (size_t i = 0; i < ps.size(); i++) {
+      if (ps[i].trivial || schedule_of(ps[i], fast) != sched) continue;
+      // tailed 4-cliques with different tail counts go to separate launches
+      const int key = (ps[i].job.cls == kTriangle && ps[i].job.kind == kTriTailedClique) ? ps[i].job.p[3] : -1;
+      const int tier = group_tier(ps[i].job);
+      bool placed = false;
+      for (auto& gr : groups) {
+        const auto& j0 = ps[gr.idx[0]].job;
+        const int k0 = (j0.cls == kTriangle && j0.kind == kTriTailedClique) ? j0.p[3] : -1;
+        if ((int)gr.idx.size() < kMaxPivotJobs && group_tier(j0) == tier && (key < 0 || k0 < 0 || key == k0)) {
+          gr.idx.push_back(i);
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) groups.push_back(Group{sched, {i}});
+    }
+    out.insert(out.end(), groups.begin(), groups.end());
+  }
+  return out;
+}
+
+int plan_group(fsgc_graph* g, const std::vector<Prepared>& ps, const Group& gr, const fsgc_tune* tune, int sample,
+               void* stream, LaunchPlan* lp, std::string* err)
+{
+  Tune t = tune_for(g, tune, gr.sched);
+  t.sample = sample;
+  std::vector<JobDesc> jobs;
+  for (size_t k : gr.idx) jobs.push_back(ps[k].job);
+  switch (gr.sched) {
+    case kPivot: return plan_wedge_pivot(jobs, &g->dg, t, stream, lp, err);
+    case kRank: return plan_triangles(jobs, &g->dg, t, stream, lp, err);
+    case kEdgePass: return plan_edgepass(jobs, &g->dg, t, stream, lp, err);
+    default: {
+      std::vector<const DeviceGraph*> graphs(jobs.size(), &g->dg);
+      return plan_jobs(jobs, graphs, t, stream, lp, err);
+    }
+  }
+}
+
 int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune, std::vector<JobResult>* res,
                  double* kernel_ms, void* stream, std::string* err, int sample)
 {
@@ -173,36 +241,18 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   }
   ms_total += ms;
   if (log_on() && !jobs.empty()) fprintf(stderr, "[fsgc] per-root launch: %zu jobs, %.1f ms\n", jobs.size(), ms);
-  // each fast-schedule launch takes up to 8 motifs that share one enumeration
   std::vector<JobResult> sres(ps.size());
-  for (int sched : {kPivot, kRank, kEdgePass}) {
-    if (rc || !fast) break;
-    std::vector<std::vector<size_t>> groups;
-    for (size_t i = 0; i < ps.size(); i++) {
-      if (ps[i].trivial || schedule_of(ps[i], fast) != sched) continue;
-      // tailed 4-cliques with different tail counts go to separate launches
-      const int key = (ps[i].job.cls == kTriangle && ps[i].job.kind == kTriTailedClique) ? ps[i].job.p[3] : -1;
-      bool placed = false;
-      for (auto& gr : groups) {
-        const auto& j0 = ps[gr[0]].job;
-        const int k0 = (j0.cls == kTriangle && j0.kind == kTriTailedClique) ? j0.p[3] : -1;
-        if ((int)gr.size() < kMaxPivotJobs && (key < 0 || k0 < 0 || key == k0)) { gr.push_back(i); placed = true; break; }
-      }
-      if (!placed) groups.push_back({i});
-    }
-    Tune t = tune_for(g, tune, sched);
-    t.sample = sample;
-    for (auto& gr : groups) {
-      std::vector<JobDesc> wj;
-      for (size_t k : gr) wj.push_back(ps[k].job);
+  if (!rc) {
+    for (const Group& gr : make_groups(ps, fast)) {
+      if (gr.sched == kPersistent) continue;  // ran above, with the automorphism jobs
+      LaunchPlan lp;
       std::vector<JobResult> r;
-      if (sched == kPivot) rc = device_run_wedge_pivot(wj, &g->dg, t, stream, &r, &ms, err);
-      else if (sched == kRank) rc = device_run_triangles(wj, &g->dg, t, stream, &r, &ms, err);
-      else rc = device_run_edgepass(wj, &g->dg, t, stream, &r, &ms, err);
+      rc = plan_group(g, ps, gr, tune, sample, stream, &lp, err);
+      if (!rc) rc = launch_run(lp, stream, &r, &ms, err);
       if (rc) break;
       ms_total += ms;
-      if (log_on()) fprintf(stderr, "[fsgc] %s launch: %zu motifs, %.1f ms\n", schedule_name(sched), gr.size(), ms);
-      for (size_t k = 0; k < r.size(); k++) sres[gr[k]] = r[k];
+      if (log_on()) fprintf(stderr, "[fsgc] %s launch: %zu motifs, %.1f ms\n", schedule_name(gr.sched), gr.idx.size(), ms);
+      for (size_t k = 0; k < r.size(); k++) sres[gr.idx[k]] = r[k];
     }
   }
   if (kernel_ms) *kernel_ms = ms_total;

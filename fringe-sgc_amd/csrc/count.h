@@ -59,6 +59,16 @@ Tune tune_for(fsgc_graph* g, const fsgc_tune* t, int sched);
 // Device scratch (bytes) one launch of `sched` takes on g with the given shape.
 size_t scratch_for(const fsgc_graph* g, int sched, const Tune& t);
 
+// The launches a set of prepared motifs needs: the per-root launch, then groups of up
+// to 8 motifs per fast schedule that share one enumeration.
+struct Group {
+  int sched;
+  std::vector<size_t> idx;  // indices into the prepared motifs
+};
+std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast);
+int plan_group(fsgc_graph* g, const std::vector<Prepared>& ps, const Group& gr, const fsgc_tune* tune, int sample,
+               void* stream, LaunchPlan* lp, std::string* err);
+
 // Runs the data-graph jobs of all prepared motifs (and the motif-graph jobs of
 // motifs whose automorphism sum is not cached) on `stream`.
 // `sample` > 1 runs only every sample-th work item (timing runs of the autotuner).

@@ -68,6 +68,17 @@ struct JobDesc {
   int8_t lev_reg[kMaxPlanLevels];  // region bitmask of the level
   int8_t lev_last[kMaxPlanLevels]; // 1 if the region is the last (top) one for this set
   int8_t lev_final[kMaxPlanLevels];// 1 if this is the last level of the whole plan
+  // general core, the same count as a dynamic programme over the Venn regions
+  // (dp_ns > 0): a state is the number of fringes already placed per anchor set,
+  // numbered in mixed radix (set i has weight prod_{j<i} (f_j + 1))
+  int32_t dp_ns;                   // states (<= 64); 0: evaluate by the plan above
+  int32_t dp_nreg;                 // regions some anchor set may draw from
+  int8_t dp_reg[64];               // their anchor bitmasks
+  int32_t dp_maxj[64];             // most fringes a region can give (sum of f_i of the sets it serves)
+  int32_t dp_sum[64];              // per state: fringes placed
+  uint64_t dp_dig[64];             // per state: the count of set i in byte i
+  uint64_t dp_mult[64];            // per state: (sum t_i)! / prod t_i!
+  uint64_t dp_allow[64];           // per region: 0xff in byte i if set i may draw from it
   Accum* out;
 };
 

@@ -1,5 +1,7 @@
 // Motif planning (host) and the device driver interface.
 #pragma once
+#include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,8 @@ struct DeviceGraph {
 
 struct DevScratch;
 
+
+
 struct JobResult {
   u128 sum = 0;
   uint64_t matches = 0;
@@ -65,11 +69,43 @@ struct JobResult {
   double index_ms = 0;    // per-graph index build time spent in this call
 };
 
+// One kernel launch prepared on the host and in device memory (work items, job
+// descriptors and scratch resident). Running it only zeroes its accumulators (a fill
+// kernel), launches the kernel and copies the accumulators to pinned host memory, so
+// a sequence of runs can be captured into a HIP graph and replayed. Scratch the
+// kernel dirties is left clean by the kernel itself (the pivot D arrays, the rank
+// marks), so replays need no other reset.
+struct LaunchPlan {
+  int nj = 0;
+  Accum* d_acc = nullptr;    // nj accumulators, then the work-queue head
+  size_t acc_bytes = 0;      // bytes zeroed per run (multiple of 8)
+  Accum* h_acc = nullptr;    // pinned host copy of the accumulators
+  double index_ms = 0;       // per-graph index build spent while planning
+  std::function<void(void* stream)> kernel;
+  std::shared_ptr<DevScratch> scratch;
+  LaunchPlan() = default;
+  LaunchPlan(const LaunchPlan&) = delete;
+  LaunchPlan& operator=(const LaunchPlan&) = delete;
+  LaunchPlan(LaunchPlan&& o) noexcept { *this = std::move(o); }
+  LaunchPlan& operator=(LaunchPlan&& o) noexcept;
+  ~LaunchPlan();
+};
+// allocates the accumulators (device and pinned host) of a plan for nj jobs
+int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err);
+// zero + kernel (+ timing events when given) + copy of the results; no allocation, no sync
+int launch_enqueue(const LaunchPlan& lp, void* stream, void* ev_begin, void* ev_end, std::string* err);
+// results of a finished run
+void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out);
+// enqueue, wait, collect
+int launch_run(const LaunchPlan& lp, void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
+
 // pivot.hip
 int device_derived_build(DeviceGraph* g, void* stream, double* ms, std::string* err);
 void device_derived_free(DeviceGraph* g);
 constexpr int kMaxPivotJobs = 8;
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t);
+int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream, LaunchPlan* lp,
+                     std::string* err);
 int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                            std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
@@ -79,6 +115,10 @@ bool triangle_schedule_ok(const JobDesc& j);
 bool edgepass_schedule_ok(const JobDesc& j);
 size_t triangle_scratch_bytes(const DeviceGraph& g, const Tune& t);
 size_t edgepass_scratch_bytes(const DeviceGraph& g, const Tune& t);
+int plan_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream, LaunchPlan* lp,
+                  std::string* err);
+int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream, LaunchPlan* lp,
+                   std::string* err);
 int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                         std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
@@ -92,6 +132,8 @@ void device_graph_free(DeviceGraph* dg);
 // Runs the given jobs (each bound to a graph) in one persistent launch on `stream`
 // and returns one result per job. kernel_ms receives the device time of the launch.
 size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t);
+int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t, void* stream,
+              LaunchPlan* lp, std::string* err);
 int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
                     void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 

@@ -373,6 +373,56 @@ __device__ void coop_commonx(const G& g, int nodes, int nonanch, const int* st, 
   __builtin_amdgcn_wave_barrier();
 }
 
+// The fringe count of one general-core match by the region DP of the job's tables,
+// wave-cooperative: lane t holds state t. `venn` is the match's Venn vector and `tab`
+// 128 u128 of LDS per wave. Returns the same value as functionx on every lane; a
+// negative region that an anchor set may draw from makes it 0, as the reference's
+// nCk(n < 0, k) does.
+__device__ u128 functionx_dp(const JobDesc* __restrict__ J, const int* venn, u128* tab, uint32_t& fl)
+{
+  const int lane = lane_id();
+  const int ns = J->dp_ns, nreg = J->dp_nreg;
+  bool neg = false;
+  for (int r = 0; r < nreg; r++) neg = neg || venn[J->dp_reg[r]] < 0;
+  if (neg) return 0;
+  constexpr ull kGuard = 0x8080808080808080ull;
+  const ull dig_t = lane < ns ? J->dp_dig[lane] : 0;
+  u128 cur = lane == 0 ? (u128)1 : (u128)0;
+  u128* old = tab;
+  u128* bin = tab + 64;
+  for (int r = 0; r < nreg; r++) {
+    const int nr = venn[J->dp_reg[r]];
+    if (nr == 0) continue;
+    const int maxj = J->dp_maxj[r];
+    old[lane] = cur;
+    bin[lane] = lane <= maxj ? ncr(nr, lane, fl) : (u128)0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const ull allow = J->dp_allow[r];
+    u128 acc = 0;
+    for (int d = 0; d < ns; d++) {
+      const ull dd = J->dp_dig[d];
+      if (dd & ~allow) continue;
+      if (J->dp_sum[d] > maxj) continue;
+      if (lane < ns && (((dig_t | kGuard) - dd) & kGuard) == kGuard) {
+        const u128 coef = mulx(bin[J->dp_sum[d]], (u128)J->dp_mult[d], fl);
+        acc = addx(acc, mulx(old[lane - d], coef, fl), fl);
+      }
+    }
+    cur = acc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  old[lane] = cur;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const u128 res = old[ns - 1];
+  __builtin_amdgcn_wave_barrier();
+  return res;
+}
+
 struct Stack6 { int v[6]; };
 
 // FC:489-564 (general_core_level<L>)
@@ -416,7 +466,14 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
           if (lane_id() == w2) myslot = slot;
           slot++;
         }
-        if (myslot >= 0) {
+        if (J->dp_ns > 0) {
+          // polynomial region DP, one staged match at a time across the wave
+          u128* tab = (u128*)(slots + kVennSlots * 64);
+          for (int q = 0; q < slot; q++) {
+            const u128 v = functionx_dp(J, slots + q * 64, tab, acc.fl);
+            if (lane_id() == 0) { acc.add(v); acc.matches++; }
+          }
+        } else if (myslot >= 0) {
           int venn[64];
           for (int q = 0; q < 64; q++) venn[q] = slots[myslot * 64 + q];
           acc.add(functionx(J, venn, acc.fl));
@@ -463,7 +520,8 @@ __device__ void do_general(const JobDesc* __restrict__ J, const WorkItem& it, La
 __global__ __launch_bounds__(kBlock) void k_count(const JobDesc* __restrict__ jobs, const WorkItem* __restrict__ items,
                                                   int nitems, int* __restrict__ head)
 {
-  __shared__ int s_venn[kWavesPerBlock][kVennSlots * 64];
+  // per wave: kVennSlots staged Venn vectors, then 128 u128 of region-DP state
+  __shared__ __attribute__((aligned(16))) int s_venn[kWavesPerBlock][kVennSlots * 64 + 128 * 4];
   int* slots = s_venn[threadIdx.x / kWave];
   LaneAcc acc;
   for (;;) {
@@ -482,6 +540,14 @@ __global__ __launch_bounds__(kBlock) void k_count(const JobDesc* __restrict__ jo
     }
     flush(acc, J->out);
   }
+}
+
+// zeroes n 64-bit words (the accumulators of a launch; a kernel rather than a memset
+// so that a captured graph holds no memset node)
+__global__ void k_zero64(ull* __restrict__ p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 }  // namespace dev
@@ -512,6 +578,78 @@ int device_init(int device, std::string* err)
     uint64_t keep = ~0ull;
     (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
   }
+  return 0;
+}
+
+LaunchPlan& LaunchPlan::operator=(LaunchPlan&& o) noexcept
+{
+  if (this != &o) {
+    if (h_acc) (void)hipHostFree(h_acc);
+    nj = o.nj; d_acc = o.d_acc; acc_bytes = o.acc_bytes; h_acc = o.h_acc; index_ms = o.index_ms;
+    kernel = std::move(o.kernel);
+    scratch = std::move(o.scratch);
+    o.h_acc = nullptr;
+    o.d_acc = nullptr;
+  }
+  return *this;
+}
+
+LaunchPlan::~LaunchPlan()
+{
+  if (h_acc) (void)hipHostFree(h_acc);
+}
+
+int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err)
+{
+  if (!lp->scratch) lp->scratch = std::make_shared<DevScratch>((hipStream_t)stream);
+  lp->nj = nj;
+  lp->acc_bytes = sizeof(Accum) * nj + 256;  // the work-queue head lives past the accumulators
+  HIP_TRY(lp->scratch->get(&lp->d_acc, lp->acc_bytes));
+  HIP_TRY(hipHostMalloc((void**)&lp->h_acc, sizeof(Accum) * std::max(1, nj), hipHostMallocDefault));
+  return 0;
+}
+
+int launch_enqueue(const LaunchPlan& lp, void* stream_v, void* ev_begin, void* ev_end, std::string* err)
+{
+  hipStream_t stream = (hipStream_t)stream_v;
+  const int words = (int)(lp.acc_bytes / 8);
+  hipLaunchKernelGGL(dev::k_zero64, dim3((words + 255) / 256), dim3(256), 0, stream, (dev::ull*)lp.d_acc, words);
+  HIP_TRY(hipGetLastError());
+  if (ev_begin) HIP_TRY(hipEventRecord((hipEvent_t)ev_begin, stream));
+  if (lp.kernel) {
+    lp.kernel(stream);
+    HIP_TRY(hipGetLastError());
+  }
+  if (ev_end) HIP_TRY(hipEventRecord((hipEvent_t)ev_end, stream));
+  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, sizeof(Accum) * lp.nj, hipMemcpyDeviceToHost, stream));
+  return 0;
+}
+
+void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out)
+{
+  out->assign(lp.nj, JobResult());
+  for (int k = 0; k < lp.nj; k++) {
+    JobResult& r = (*out)[k];
+    r.sum = ((unsigned __int128)lp.h_acc[k].hi << 64) | lp.h_acc[k].lo;
+    r.matches = lp.h_acc[k].matches;
+    r.flags = lp.h_acc[k].flags;
+    r.bytes = lp.h_acc[k].bytes;
+    r.index_ms = (k == 0) ? lp.index_ms : 0;
+  }
+}
+
+int launch_run(const LaunchPlan& lp, void* stream_v, std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  hipStream_t stream = (hipStream_t)stream_v;
+  EventPair ev;
+  HIP_TRY(ev.create());
+  int rc = launch_enqueue(lp, stream_v, ev.a, ev.b, err);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+  if (kernel_ms) *kernel_ms = ms;
+  launch_collect(lp, out);
   return 0;
 }
 
@@ -600,14 +738,11 @@ size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t)
   return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 64) + 4096;
 }
 
-int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
-                    void* stream_v, std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
+              void* stream_v, LaunchPlan* lp, std::string* err)
 {
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
-  out->assign(nj, JobResult());
-  if (kernel_ms) *kernel_ms = 0;
-  if (nj == 0) return 0;
   std::vector<WorkItem> items;
   for (int k = 0; k < nj; k++) {
     jobs[k].idx = graphs[k]->idx;
@@ -617,49 +752,41 @@ int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGr
     cut_items(jobs[k], *graphs[k], k, t, &items);
   }
   sample_items(&items, t.sample);
-  // most expensive items first: order by job, big graphs first
-  DevScratch sc(stream);
-  Accum* d_acc = nullptr;
+  int rc = plan_accumulators(lp, nj, stream, err);
+  if (rc) return rc;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
-  int* d_head = nullptr;
-  const size_t acc_bytes = sizeof(Accum) * nj;
-  HIP_TRY(sc.get(&d_acc, acc_bytes + 256));
-  d_head = (int*)((char*)d_acc + acc_bytes + 128);
-  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(sc.get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
-  HIP_TRY(hipMemsetAsync(d_acc, 0, acc_bytes + 256, stream));
-  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
+  HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
+  HIP_TRY(lp->scratch->get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+  if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
   if (!items.empty())
     HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, stream));
+  // the host arrays may go away before the copies run
+  HIP_TRY(hipStreamSynchronize(stream));
   int dev = 0, cus = 256;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int grid = t.grid > 0 ? t.grid : cus * 8;
   const int nitems = (int)items.size();
   grid = std::max(1, std::min(grid, (nitems + dev::kWavesPerBlock - 1) / dev::kWavesPerBlock));
-  EventPair ev;
-  HIP_TRY(ev.create());
-  HIP_TRY(hipEventRecord(ev.a, stream));
-  if (nitems > 0) {
-    hipLaunchKernelGGL(dev::k_count, dim3(grid), dim3(dev::kBlock), 0, stream, d_jobs, d_items, nitems, d_head);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(ev.b, stream));
-  std::vector<Accum> h_acc(nj);
-  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, acc_bytes, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (kernel_ms) *kernel_ms = ms;
-  for (int k = 0; k < nj; k++) {
-    JobResult& r = (*out)[k];
-    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
-    r.matches = h_acc[k].matches;
-    r.flags = h_acc[k].flags;
-  }
+  if (nitems > 0)
+    lp->kernel = [=](void* s) {
+      hipLaunchKernelGGL(dev::k_count, dim3(grid), dim3(dev::kBlock), 0, (hipStream_t)s, d_jobs, d_items, nitems, d_head);
+    };
   return 0;
+}
+
+int device_run_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
+                    void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  out->assign(jobs.size(), JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (jobs.empty()) return 0;
+  LaunchPlan lp;
+  int rc = plan_jobs(jobs, graphs, t, stream, &lp, err);
+  return rc ? rc : launch_run(lp, stream, out, kernel_ms, err);
 }
 
 }  // namespace fsgc

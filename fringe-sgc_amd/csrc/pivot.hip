@@ -466,18 +466,14 @@ size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
   return sizeof(unsigned) * grid * (dstride + estride * dev::kPivWaves) + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
 }
 
-// Runs up to kMaxPivotJobs wedge-class jobs on one graph in one pivot-schedule launch.
-int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
-                           std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+// Prepares one pivot-schedule launch of up to kMaxPivotJobs wedge-class jobs.
+int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v, LaunchPlan* lp,
+                     std::string* err)
 {
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
-  out->assign(nj, JobResult());
-  if (kernel_ms) *kernel_ms = 0;
-  if (nj == 0) return 0;
   if (nj > dev::kMaxPivotJobs) { if (err) *err = "too many jobs for one pivot launch"; return -7; }
-  double index_ms = 0;
-  int rc = device_derived_build(g, stream_v, &index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
   if (rc) return rc;
   bool need_c3 = false;
   for (auto& j : jobs) {
@@ -509,26 +505,26 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
   const int grid = std::max(1, std::min<int>(pivot_grid(t), (int)sorted.size()));
   const long long dstride = ((long long)g->n + 63) / 64 * 64;
   const long long estride = ((long long)g->maxdeg + 63) / 64 * 64;
-  DevScratch sc(stream);
+  rc = plan_accumulators(lp, nj, stream, err);
+  if (rc) return rc;
+  DevScratch& sc = *lp->scratch;
   unsigned* dbuf = nullptr;
   unsigned* ebuf = nullptr;
-  Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
   HIP_TRY(sc.get(&ebuf, sizeof(unsigned) * estride * grid * dev::kPivWaves));
-  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
-  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
-  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
+  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
+  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(sc.get(&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
+  // D starts at zero; every launch leaves it zero again (phase C of the kernel)
   HIP_TRY(hipMemsetAsync(dbuf, 0, sizeof(unsigned) * dstride * grid, stream));
-  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
-  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice, stream));
-  EventPair ev;
-  HIP_TRY(ev.create());
-  HIP_TRY(hipEventRecord(ev.a, stream));
+  if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  if (!sorted.empty())
+    HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice,
+                           stream));
+  HIP_TRY(hipStreamSynchronize(stream));  // the host arrays go out of scope
   const int nitems = (int)sorted.size();
   const int c3 = need_c3 ? 1 : 0;
   bool full = false;  // any fringe vector with a multi-vertex anchor set
@@ -536,34 +532,37 @@ int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tun
     if (j.kind == kWedgeGeneral)
       for (int r = 0; r < j.rot; r++)
         full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
-  if (nitems > 0) {
-#define FSGC_PIVOT_LAUNCH(NJ, FULL)                                                                             \
-  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL>), dim3(grid), dim3(dev::kPivBlock), 0, stream, g->idx, g->adj, \
-                     g->tsup, g->tl_off, g->tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
+  if (nitems == 0) return 0;
+  const int* idx = g->idx;
+  const int* adj = g->adj;
+  const int* tsup = g->tsup;
+  const long long* tl_off = g->tl_off;
+  const int* tl = g->tl;
+  lp->kernel = [=](void* sv) {
+    hipStream_t s = (hipStream_t)sv;
+#define FSGC_PIVOT_LAUNCH(NJ, FULL)                                                                           \
+  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL>), dim3(grid), dim3(dev::kPivBlock), 0, s, idx, adj, tsup,    \
+                     tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
     if (nj == 1) {
       if (full) FSGC_PIVOT_LAUNCH(1, true); else FSGC_PIVOT_LAUNCH(1, false);
     } else {
       if (full) FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, true); else FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, false);
     }
 #undef FSGC_PIVOT_LAUNCH
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(ev.b, stream));
-  std::vector<Accum> h_acc(nj);
-  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (kernel_ms) *kernel_ms = ms;
-  for (int k = 0; k < nj; k++) {
-    JobResult& r = (*out)[k];
-    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
-    r.matches = h_acc[k].matches;
-    r.flags = h_acc[k].flags;
-    r.bytes = h_acc[k].bytes;
-    r.index_ms = (k == 0) ? index_ms : 0;
-  }
+  };
   return 0;
+}
+
+// Runs up to kMaxPivotJobs wedge-class jobs on one graph in one pivot-schedule launch.
+int device_run_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                           std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  out->assign(jobs.size(), JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (jobs.empty()) return 0;
+  LaunchPlan lp;
+  int rc = plan_wedge_pivot(jobs, g, t, stream, &lp, err);
+  return rc ? rc : launch_run(lp, stream, out, kernel_ms, err);
 }
 
 }  // namespace fsgc

@@ -93,6 +93,57 @@ int general_quirk_factor(const MatchingOrder& mo)
   return (mo.anch[mo.anchorsets - 1] == anchors) ? mo.fcnt[mo.anchorsets - 1] + 1 : 1;
 }
 
+// Tables of the region DP for a general core: sets S_i = anch[i] with f_i fringes;
+// the count of disjoint choices F_i within the union of the regions R >= S_i (the
+// quantity functionX sums, FC:408-486) is, region by region,
+//   new[t] = sum_{d <= t, d_i = 0 unless S_i <= R} old[t - d] * C(n_R, |d|) * |d|! / prod d_i!
+// starting from old = [1, 0, ...] and read at t = (f_1, ..., f_A).
+static void build_region_dp(const MatchingOrder& mo, int top, JobDesc* j)
+{
+  j->dp_ns = 0;
+  j->dp_nreg = 0;
+  const int A = mo.anchorsets;
+  if (A < 1 || A > 8) return;
+  long long ns = 1;
+  for (int i = 0; i < A; i++) {
+    if (mo.fcnt[i] < 1 || mo.fcnt[i] > 63) return;
+    ns *= mo.fcnt[i] + 1;
+    if (ns > 64) return;
+  }
+  for (int t = 0; t < ns; t++) {
+    int rest = t, sum = 0;
+    uint64_t dig = 0;
+    unsigned __int128 mult = 1;
+    for (int i = 0; i < A; i++) {
+      const int ti = rest % (mo.fcnt[i] + 1);
+      rest /= mo.fcnt[i] + 1;
+      dig |= (uint64_t)ti << (8 * i);
+      // multinomial as a product of binomials C(sum + ti, ti)
+      for (int q = 1; q <= ti; q++) mult = mult * (unsigned)(sum + q) / (unsigned)q;
+      sum += ti;
+      if (mult >> 64) return;
+    }
+    j->dp_dig[t] = dig;
+    j->dp_sum[t] = sum;
+    j->dp_mult[t] = (uint64_t)mult;
+  }
+  int nr = 0;
+  for (int R = 1; R <= top; R++) {
+    if ((R & ~top) != 0) continue;
+    uint64_t allow = 0;
+    int maxj = 0;
+    for (int i = 0; i < A; i++)
+      if ((mo.anch[i] & ~R) == 0) { allow |= 0xffull << (8 * i); maxj += mo.fcnt[i]; }
+    if (!allow) continue;
+    j->dp_reg[nr] = (int8_t)R;
+    j->dp_allow[nr] = allow;
+    j->dp_maxj[nr] = std::min(maxj, 63);
+    nr++;
+  }
+  j->dp_nreg = nr;
+  j->dp_ns = (int)ns;
+}
+
 int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
 {
   int rc = validate_mo(mo, err);
@@ -185,6 +236,7 @@ int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
         }
       }
       j->nlev = L;
+      build_region_dp(mo, top, j);
       break;
     }
   }

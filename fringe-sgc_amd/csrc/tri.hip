@@ -300,18 +300,14 @@ size_t edgepass_scratch_bytes(const DeviceGraph& g, const Tune& t)
   return sizeof(WorkItem) * ((size_t)g.m / step + 2) + 4096;
 }
 
-// Runs up to kMaxTriJobs closed-form edge and star jobs in one sweep over the edges.
-int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
-                        std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+// Prepares one sweep over the edges for up to kMaxTriJobs closed-form edge and star jobs.
+int plan_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v, LaunchPlan* lp,
+                  std::string* err)
 {
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
-  out->assign(nj, JobResult());
-  if (kernel_ms) *kernel_ms = 0;
-  if (nj == 0) return 0;
   if (nj > dev::kMaxTriJobs) { if (err) *err = "too many jobs for one edge pass"; return -7; }
-  double index_ms = 0;
-  int rc = device_derived_build(g, stream_v, &index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
   if (rc) return rc;
   for (auto& j : jobs) { j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m; }
   std::vector<WorkItem> items;
@@ -329,56 +325,52 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int grid = t.grid > 0 ? t.grid : cus * 8;
   grid = std::max(1, std::min<int>(grid, ((int)items.size() + 3) / 4));
-  DevScratch sc(stream);
-  Accum* d_acc = nullptr;
+  rc = plan_accumulators(lp, nj, stream, err);
+  if (rc) return rc;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
-  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
-  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
-  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(sc.get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
-  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
-  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, stream));
-  EventPair ev;
-  HIP_TRY(ev.create());
-  HIP_TRY(hipEventRecord(ev.a, stream));
-  if (!items.empty()) {
+  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
+  HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
+  HIP_TRY(lp->scratch->get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+  if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  if (!items.empty())
+    HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));  // the host arrays go out of scope
+  const int nitems = (int)items.size();
+  if (nitems == 0) return 0;
+  const int* idx = g->idx;
+  const int* adj = g->adj;
+  const int* tsup = g->tsup;
+  lp->kernel = [=](void* sv) {
+    hipStream_t s = (hipStream_t)sv;
     if (nj == 1)
-      hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, stream, g->idx, g->adj, g->tsup, d_jobs, nj,
-                         d_items, (int)items.size(), d_head);
+      hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, d_jobs, nj, d_items,
+                         nitems, d_head);
     else
-      hipLaunchKernelGGL((dev::k_edge_pass<dev::kMaxTriJobs>), dim3(grid), dim3(256), 0, stream, g->idx, g->adj,
-                         g->tsup, d_jobs, nj, d_items, (int)items.size(), d_head);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(ev.b, stream));
-  std::vector<Accum> h_acc(nj);
-  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (kernel_ms) *kernel_ms = ms;
-  for (int k = 0; k < nj; k++) {
-    JobResult& r = (*out)[k];
-    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
-    r.matches = h_acc[k].matches;
-    r.flags = h_acc[k].flags;
-    r.index_ms = (k == 0) ? index_ms : 0;
-  }
+      hipLaunchKernelGGL((dev::k_edge_pass<dev::kMaxTriJobs>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, d_jobs,
+                         nj, d_items, nitems, d_head);
+  };
   return 0;
 }
 
-// Runs up to kMaxTriJobs triangle-based jobs (same graph) in one rank-schedule launch.
-int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v,
-                         std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                        std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  out->assign(jobs.size(), JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (jobs.empty()) return 0;
+  LaunchPlan lp;
+  int rc = plan_edgepass(jobs, g, t, stream, &lp, err);
+  return rc ? rc : launch_run(lp, stream, out, kernel_ms, err);
+}
+
+// Prepares one rank-schedule launch of up to kMaxTriJobs triangle-based jobs.
+int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream_v, LaunchPlan* lp,
+                   std::string* err)
 {
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
-  out->assign(nj, JobResult());
-  if (kernel_ms) *kernel_ms = 0;
-  if (nj == 0) return 0;
   if (nj > dev::kMaxTriJobs) { if (err) *err = "too many jobs for one triangle launch"; return -7; }
   int tails = -1;
   for (auto& j : jobs) {
@@ -387,8 +379,7 @@ int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune&
       tails = j.p[3];
     }
   }
-  double index_ms = 0;
-  int rc = device_derived_build(g, stream_v, &index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
   if (rc) return rc;
   bool need_c3 = false, need_lt = false, full = false;
   for (auto& j : jobs) {
@@ -412,54 +403,55 @@ int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune&
   int grid = triangle_grid(t);
   grid = std::max(1, std::min<int>(grid, ((int)items.size() + dev::kTriWaves - 1) / dev::kTriWaves));
   const long long mstride = ((long long)g->maxdeg + 63) / 64 * 64;
-  DevScratch sc(stream);
+  rc = plan_accumulators(lp, nj, stream, err);
+  if (rc) return rc;
   unsigned char* gmark = nullptr;
-  Accum* d_acc = nullptr;
   JobDesc* d_jobs = nullptr;
   dev::TriItem* d_items = nullptr;
-  HIP_TRY(sc.get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
-  HIP_TRY(sc.get(&d_acc, sizeof(Accum) * nj + 256));
-  int* d_head = (int*)((char*)d_acc + sizeof(Accum) * nj + 128);
-  for (int k = 0; k < nj; k++) jobs[k].out = d_acc + k;
-  HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * nj));
-  HIP_TRY(sc.get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+  HIP_TRY(lp->scratch->get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
+  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
+  for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
+  HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
+  HIP_TRY(lp->scratch->get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+  // marks start at zero; every launch unmarks what it marked
   HIP_TRY(hipMemsetAsync(gmark, 0, (size_t)mstride * grid * dev::kTriWaves, stream));
-  HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum) * nj + 256, stream));
-  HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice, stream));
-  EventPair ev;
-  HIP_TRY(ev.create());
-  HIP_TRY(hipEventRecord(ev.a, stream));
+  if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
+  if (!items.empty())
+    HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));  // the host arrays go out of scope
   const int nitems = (int)items.size();
-  if (nitems > 0) {
+  if (nitems == 0) return 0;
+  const int* idx = g->idx;
+  const int* adj = g->adj;
+  const int* tsup = g->tsup;
+  const long long* tl_off = g->tl_off;
+  const int* tl = g->tl;
+  const int c3 = need_c3 ? 1 : 0, lt = need_lt ? 1 : 0, tl_arg = std::max(0, tails);
+  lp->kernel = [=](void* sv) {
+    hipStream_t s = (hipStream_t)sv;
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
-  hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, stream, g->idx,     \
-                     g->adj, g->tsup, g->tl_off, g->tl, d_jobs, nj, need_c3 ? 1 : 0, need_lt ? 1 : 0,           \
-                     std::max(0, tails), d_items,                                                               \
-                     nitems, d_head, gmark, mstride)
+  hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, s, idx, adj, tsup,  \
+                     tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items, nitems, d_head, gmark, mstride)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
     } else {
       if (full) FSGC_TRI_LAUNCH(dev::kMaxTriJobs, true); else FSGC_TRI_LAUNCH(dev::kMaxTriJobs, false);
     }
 #undef FSGC_TRI_LAUNCH
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(ev.b, stream));
-  std::vector<Accum> h_acc(nj);
-  HIP_TRY(hipMemcpyAsync(h_acc.data(), d_acc, sizeof(Accum) * nj, hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (kernel_ms) *kernel_ms = ms;
-  for (int k = 0; k < nj; k++) {
-    JobResult& r = (*out)[k];
-    r.sum = ((unsigned __int128)h_acc[k].hi << 64) | h_acc[k].lo;
-    r.matches = h_acc[k].matches;
-    r.flags = h_acc[k].flags;
-    r.index_ms = (k == 0) ? index_ms : 0;
-  }
+  };
   return 0;
+}
+
+// Runs up to kMaxTriJobs triangle-based jobs (same graph) in one rank-schedule launch.
+int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
+                         std::vector<JobResult>* out, double* kernel_ms, std::string* err)
+{
+  out->assign(jobs.size(), JobResult());
+  if (kernel_ms) *kernel_ms = 0;
+  if (jobs.empty()) return 0;
+  LaunchPlan lp;
+  int rc = plan_triangles(jobs, g, t, stream, &lp, err);
+  return rc ? rc : launch_run(lp, stream, out, kernel_ms, err);
 }
 
 }  // namespace fsgc

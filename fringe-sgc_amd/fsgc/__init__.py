@@ -89,6 +89,9 @@ def _load():
         "fsgc_free_host": (None, [C.c_void_p]),
         "fsgc_random_motifs": (C.c_int, [C.c_uint64, C.c_int, C.c_char_p]),
         "fsgc_random_patterns": (C.c_int, [C.c_uint64, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "fsgc_batch_create": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P(Tune), P(C.c_void_p)]),
+        "fsgc_batch_run": (C.c_int, [C.c_void_p, P(_Result), P(C.c_double)]),
+        "fsgc_batch_free": (None, [C.c_void_p]),
         "fsgc_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int32, P(TuneReport), P(C.c_int32)]),
         "fsgc_graph_untune": (C.c_int, [C.c_void_p]),
         "fsgc_server_create": (C.c_int, [P(ServerConfig), P(C.c_void_p)]),
@@ -144,6 +147,20 @@ def _result(r: _Result) -> Result:
     return Result(exact=r.exact_lo | (r.exact_hi << 64), ref_compat=r.ref_compat, sg=r.sg_lo | (r.sg_hi << 64),
                   aut=r.aut, core_matches=r.core_matches, flags=r.flags, motif_class=r.motif_class,
                   kernel_ms=r.kernel_ms, wall_ms=r.wall_ms, alg_bytes=r.alg_bytes, index_ms=r.index_ms)
+
+
+def motif_class(mo: bytes) -> str:
+    """Class of a matching order by its core (fringeCount.cu:2244-2252): 1 vertex,
+    2 edge, 3 wedge or triangle (list[1] names one or two earlier core vertices),
+    4-6 general."""
+    nodes = mo[0]
+    if nodes <= 1:
+        return "vertex"
+    if nodes == 2:
+        return "edge"
+    if nodes == 3:
+        return "wedge" if bin(mo[9]).count("1") == 1 else "triangle"
+    return "general"
 
 
 def init(device: int = 0):
@@ -204,6 +221,12 @@ class Graph:
         _check(lib.fsgc_count_batch(self._h, buf, k, C.byref(tune) if tune else None, out, C.byref(ms)))
         return [_result(out[i]) for i in range(k)], ms.value
 
+    def capture_batch(self, mos: list[bytes], tune: Tune | None = None) -> "CapturedBatch":
+        """The motifs' launches captured once as a HIP graph (replay with .run())."""
+        h = C.c_void_p()
+        _check(lib.fsgc_batch_create(self._h, b"".join(mos), len(mos), C.byref(tune) if tune else None, C.byref(h)))
+        return CapturedBatch(h, len(mos), self)
+
     def autotune(self, mos: list[bytes], sample: int = 16) -> list[dict]:
         """Times launch shapes per schedule on every sample-th work item and keeps the
         fastest for this graph (used by later counts for fields left at zero)."""
@@ -224,6 +247,32 @@ class Graph:
     def free(self):
         if self._h:
             lib.fsgc_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CapturedBatch:
+    """A batch replayed from one captured HIP graph."""
+
+    def __init__(self, handle, k: int, graph: Graph):
+        self._h = handle
+        self._k = k
+        self._graph = graph  # the graph must outlive the batch
+
+    def run(self):
+        out = (_Result * max(1, self._k))()
+        ms = C.c_double()
+        _check(lib.fsgc_batch_run(self._h, out, C.byref(ms)))
+        return [_result(out[i]) for i in range(self._k)], ms.value
+
+    def free(self):
+        if self._h:
+            lib.fsgc_batch_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -13,6 +13,7 @@
  *   fsgc_graph_free      freeECLgraph + cudaFree              src/ECLgraph.h:88-96, src/fringeCount.cu:2325-2329
  *   fsgc_count           occurrences(mo, d_g, wl, ...)         src/fringeCount.cu:2199-2262
  *   fsgc_count_batch     (no counterpart: the reference counts one motif per process)
+ *   fsgc_batch_*         (no counterpart: the same, captured once as a HIP graph and replayed)
  *   fsgc_preprocess      fringePreprocess n s d s d ...        src/fringePreprocess.cpp:287-365
  *   fsgc_preprocess_pattern  fringePreprocess <pattern#>       src/fringePreprocess.cpp:296-298, src/patterns.h:57-354
  *   fsgc_print_matching_order  printMatchingOrder              src/fringes.h:61-98
@@ -95,6 +96,17 @@ int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* 
    fsgc_count of mos + i*139 alone. batch_ms receives the device time of the launch. */
 int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* tune, fsgc_result* out,
                      double* batch_ms);
+
+/* A batch captured as one HIP graph: fsgc_batch_create plans every launch the k motifs
+   need on g (work items, descriptors and scratch resident; the graph's edge index and
+   the automorphism sums computed by one eager run) and captures the zeroing, the
+   kernels and the result copies into a HIP graph; fsgc_batch_run replays it with one
+   launch and one synchronisation. out[i] equals fsgc_count of motif i. g must outlive
+   the batch; tune may be NULL (slices and the per-root schedule are refused). */
+typedef struct fsgc_batch fsgc_batch;
+int fsgc_batch_create(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* tune, fsgc_batch** out);
+int fsgc_batch_run(fsgc_batch* b, fsgc_result* out, double* graph_ms);
+void fsgc_batch_free(fsgc_batch* b);
 
 /* Autotuner. For every schedule the given motifs use (per-root kernel, pivot, rank,
    edge pass), times candidate launch shapes (workgroups, roots per work item, hub

@@ -85,3 +85,21 @@ def test_random_motif_classes():
     mos = fsgc.random_motifs(3, 64)
     classes = [min(4, {1: 0, 2: 1}.get(m[0], 2 if m[0] == 3 and bin(m[9]).count("1") == 1 else (3 if m[0] == 3 else 4))) for m in mos]
     assert set(classes) == {0, 1, 2, 3, 4}
+
+
+def test_motif_class_helper_matches_oracle():
+    """fsgc.motif_class (host-side, from the matching order) names the class the
+    oracle dispatches on, for the C3 generator's motifs."""
+    import oracle
+    ni, nl = fsgc.generate_csr("er:12:30:1")
+    for mo in fsgc.random_motifs(3, 64):
+        o = oracle.count(mo, ni, nl)
+        assert fsgc.CLASS_NAMES[o["motif_class"]] == fsgc.motif_class(mo)
+
+
+def test_reproduce_library_matches_reference_families():
+    """The edges/s harness finds every motif file family of the reference's library."""
+    from fsgc import reproduce
+    sizes = {f: len(reproduce.library(f)) for f in reproduce.FAMILIES}
+    assert sizes == {"vertex": 5, "edge": 6, "wedge": 4, "triangle": 4, "tails": 11, "tri_fringe": 11,
+                     "fig3wedge": 11}
