@@ -12,9 +12,13 @@ Metric: core matches per second (core embeddings whose fringe formula is evaluat
 Also reported: the roofline of the dominant kernel (k_wedge_pivot) from its
 algorithmic bytes and its HIP-event time, and a CPU baseline: the oracle (the
 reference's core+fringe method restated in C++/OpenMP) on a deterministic slice of
-root vertices, checked bit-exact against the GPU on the same slice.
+root vertices, checked bit-exact against the GPU on the same slice. Beside it, the
+two other parts of the metric on small graphs: batched vs one-motif-per-launch motifs/s
+over the random-motif generator (seed 3) and the served query stream's p50/p99.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-extras]
+    python bench.py --batch <graph spec> [--classes vertex,edge,...]   (plain vs batched)
+    python bench.py --serve <spec>,<spec> [--rate Q] [--queries N]    (served stream)
 """
 import argparse
 import json
@@ -49,6 +53,11 @@ WORKLOADS = {
 }
 
 
+# the small batched comparison and served stream reported beside the main workload
+EXTRA_BATCH = "rmat:16:16:0.57:0.19:0.19:11"
+EXTRA_SERVE = ["rmat:16:16:0.57:0.19:0.19:23", "rmat:14:32:0.57:0.19:0.19:29"]
+
+
 def physical_cores() -> int:
     try:
         import psutil
@@ -72,7 +81,7 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def batch_compare(spec, seed=3, count=64, classes=None, out_path=None):
+def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True):
     """Plain path (one fsgc_count per motif) against the batched path (one
     fsgc_count_batch over all motifs) on the same resident graph: motifs/s of each,
     per-class times, and per-motif equality of every result. `classes` restricts the
@@ -139,7 +148,7 @@ def batch_compare(spec, seed=3, count=64, classes=None, out_path=None):
         "captured_s": round(t_cap, 3), "captured_speedup": round(t_plain / t_cap, 3), "captured_identical": same_cap,
         "per_class": per_class,
     }
-    print(json.dumps(res), flush=True)
+    print(json.dumps(res) if emit else "# " + json.dumps(res), flush=True)
     if out_path:
         json.dump({"summary": res, "counts": [str(r.exact) for r in batch]}, open(out_path, "w"), indent=1)
     return res
@@ -153,7 +162,7 @@ def percentile(xs, q):
     return xs[k]
 
 
-def serve_bench(specs, rate, nq, slots, classes=None, seed=5, out_path=None, check=True):
+def serve_bench(specs, rate, nq, slots, classes=None, seed=5, out_path=None, check=True, emit=True):
     """Served query stream: graphs resident, Poisson arrivals at `rate` queries/s from
     `seed`, each query a uniform graph and the next motif of the random-motif generator
     (seed `seed`), submitted to the library's server at its arrival time. Latency is
@@ -236,7 +245,7 @@ def serve_bench(specs, rate, nq, slots, classes=None, seed=5, out_path=None, che
         "per_class_p50_p99_ms": {c: [round(percentile(v, 50), 3), round(percentile(v, 99), 3)] for c, v in by_class.items()},
         "identical_alone_vs_served": identical,
     }
-    print(json.dumps(res), flush=True)
+    print(json.dumps(res) if emit else "# " + json.dumps(res), flush=True)
     if out_path:
         json.dump(res, open(out_path, "w"), indent=1)
     return res
@@ -249,6 +258,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the small batched and served runs")
     ap.add_argument("--tune", action="store_true", help="autotune the launch shape on the workload first")
     ap.add_argument("--tune-sample", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
@@ -358,6 +368,17 @@ def main():
         if not parity:
             print("# CPU/GPU slice mismatch!", file=sys.stderr)
 
+    # the other two parts of the metric, on small graphs so that the run stays within
+    # minutes (the full-size runs: bench.py --batch / --serve, results under profiles/)
+    batched = served = None
+    if not args.no_extras:
+        b = batch_compare(EXTRA_BATCH, classes={"vertex", "edge", "wedge", "triangle"}, emit=False)
+        batched = {k: b[k] for k in ("graph", "motifs", "excluded", "plain_motifs_per_s", "batched_motifs_per_s",
+                                     "speedup", "captured_speedup", "identical", "captured_identical")}
+        sv = serve_bench(EXTRA_SERVE, 20.0, 32, 2, classes={"vertex", "edge", "wedge", "triangle"}, emit=False)
+        served = {k: sv[k] for k in ("graphs", "offered_rate_qps", "queries", "slots", "p50_ms", "p99_ms",
+                                     "motifs_per_s", "identical_alone_vs_served")}
+
     out = {
         "metric": "core_matches_per_s",
         "value": round(value, 1),
@@ -385,6 +406,8 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -31,9 +31,10 @@
 namespace fsgc {
 namespace dev {
 
-constexpr int kPivBlock = 256;
-constexpr int kPivWaves = kPivBlock / kWave;
-constexpr int kECap = 4096;  // LDS counters per wave (positions of N(u))
+// launch shapes of the pivot kernel: waves per workgroup and LDS counters per wave
+// (positions of N(u)); 4 x 4096 (64 KB, two workgroups per CU), 8 x 4096 and
+// 16 x 2048 (128 KB, one workgroup per CU with 8 or 16 waves sharing one D array)
+constexpr int kPivWavesMax = 16;
 constexpr int kMaxPivotJobs = 8;
 
 struct PivotItem {
@@ -213,8 +214,8 @@ __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u,
   return true;
 }
 
-template <int NJ, bool kFull>
-__global__ __launch_bounds__(kPivBlock) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
+template <int NJ, bool kFull, int kPivWaves, int kECap>
+__global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
                                                            const int* __restrict__ tsup,
                                                            const long long* __restrict__ tl_off,
                                                            const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
@@ -451,19 +452,24 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
 // Grid of the pivot launch (before clamping to the number of items) and the device
 // scratch it takes: one dense D array of n counters per workgroup and one spill array
 // of maxdeg counters per wave for centres whose list does not fit in LDS.
+// waves per workgroup of the pivot launch: Tune::block of 512 or 1024 threads selects
+// the 8- and 16-wave shapes, anything else the 4-wave one
+static int pivot_waves(const Tune& t) { return t.block >= 1024 ? 16 : (t.block >= 512 ? 8 : 4); }
+
 static int pivot_grid(const Tune& t)
 {
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  return t.grid > 0 ? t.grid : cus * 4;
+  // one workgroup per CU for the 128 KB shapes (two with 64 KB), twice over
+  return t.grid > 0 ? t.grid : (pivot_waves(t) > 4 ? cus * 2 : cus * 4);
 }
 
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
   const size_t grid = (size_t)pivot_grid(t);
   const size_t dstride = ((size_t)g.n + 63) / 64 * 64, estride = ((size_t)g.maxdeg + 63) / 64 * 64;
-  return sizeof(unsigned) * grid * (dstride + estride * dev::kPivWaves) + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
+  return sizeof(unsigned) * grid * (dstride + estride * pivot_waves(t)) + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
 }
 
 // Prepares one pivot-schedule launch of up to kMaxPivotJobs wedge-class jobs.
@@ -513,7 +519,8 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   JobDesc* d_jobs = nullptr;
   dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
-  HIP_TRY(sc.get(&ebuf, sizeof(unsigned) * estride * grid * dev::kPivWaves));
+  const int waves = pivot_waves(t);
+  HIP_TRY(sc.get(&ebuf, sizeof(unsigned) * estride * grid * waves));
   int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
@@ -540,14 +547,21 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const int* tl = g->tl;
   lp->kernel = [=](void* sv) {
     hipStream_t s = (hipStream_t)sv;
-#define FSGC_PIVOT_LAUNCH(NJ, FULL)                                                                           \
-  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL>), dim3(grid), dim3(dev::kPivBlock), 0, s, idx, adj, tsup,    \
-                     tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
+#define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP)                                                                   \
+  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
+                     tsup, tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
+#define FSGC_PIVOT_SHAPE(NJ, FULL)                                          \
+  do {                                                                      \
+    if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048);                 \
+    else if (waves == 8) FSGC_PIVOT_LAUNCH(NJ, FULL, 8, 4096);              \
+    else FSGC_PIVOT_LAUNCH(NJ, FULL, 4, 4096);                              \
+  } while (0)
     if (nj == 1) {
-      if (full) FSGC_PIVOT_LAUNCH(1, true); else FSGC_PIVOT_LAUNCH(1, false);
+      if (full) FSGC_PIVOT_SHAPE(1, true); else FSGC_PIVOT_SHAPE(1, false);
     } else {
-      if (full) FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, true); else FSGC_PIVOT_LAUNCH(dev::kMaxPivotJobs, false);
+      if (full) FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, true); else FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, false);
     }
+#undef FSGC_PIVOT_SHAPE
 #undef FSGC_PIVOT_LAUNCH
   };
   return 0;

@@ -1,5 +1,6 @@
-// Autotuner: per graph and schedule, the launch shape (workgroups, roots per work
-// item, hub threshold) with the shortest device time on a sample of the work items.
+// Autotuner: per graph and schedule, the launch shape (workgroup size where the kernel
+// has several, workgroups, roots per work item, hub threshold) with the shortest
+// device time on a sample of the work items.
 // The counts never depend on the shape (every shape enumerates the same roots), so
 // tuning only changes time.
 #include <hip/hip_runtime.h>
@@ -14,22 +15,25 @@
 namespace {
 
 struct Cand {
-  int grid, chunk, hub;
+  int block, grid, chunk, hub;
 };
 
 std::vector<Cand> candidates(int sched, int cus)
 {
   std::vector<Cand> c;
-  std::vector<int> grids, chunks = {64, 256, 1024}, hubs = {4096};
+  std::vector<int> grids, chunks = {64, 256, 1024}, hubs = {4096}, blocks = {256};
   switch (sched) {
-    case fsgc::kPivot: grids = {cus * 2, cus * 4, cus * 8}; break;
+    case fsgc::kPivot: grids = {cus * 2, cus * 4, cus * 8}; blocks = {256, 1024}; break;
     case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; break;
     case fsgc::kEdgePass: grids = {cus * 4, cus * 8, cus * 16}; break;
     default: grids = {cus * 4, cus * 8, cus * 16}; hubs = {1024, 4096}; break;
   }
-  for (int g : grids)
-    for (int ch : chunks)
-      for (int h : hubs) c.push_back(Cand{g, ch, h});
+  for (int b : blocks)
+    for (int g : grids) {
+      if (b > 256 && g > cus * 4) continue;  // one 1024-thread workgroup per CU fits; more only queue
+      for (int ch : chunks)
+        for (int h : hubs) c.push_back(Cand{b, g, ch, h});
+    }
   return c;
 }
 
@@ -90,18 +94,19 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     int ncand = 1;
     for (const Cand& c : candidates(s, cus)) {
       fsgc_tune ft = def;
+      ft.block = c.block;
       ft.grid = c.grid;
       ft.chunk = c.chunk;
       ft.hub_degree = c.hub;
       fsgc::Tune t;
-      t.grid = c.grid; t.chunk = c.chunk; t.hub_degree = c.hub;
+      t.block = c.block; t.grid = c.grid; t.chunk = c.chunk; t.hub_degree = c.hub;
       if (fsgc::scratch_for(g, s, t) > free_b / 4) continue;  // leave room for everything else
       rc = time_shape(ft, &ms);
       if (rc) { fsgc::set_error(err); return rc; }
       ncand++;
       if (fsgc::log_on())
-        fprintf(stderr, "[fsgc] tune %s grid %d chunk %d hub %d: %.2f ms\n", fsgc::schedule_name(s), c.grid, c.chunk,
-                c.hub, ms);
+        fprintf(stderr, "[fsgc] tune %s block %d grid %d chunk %d hub %d: %.2f ms\n", fsgc::schedule_name(s), c.block,
+                c.grid, c.chunk, c.hub, ms);
       if (ms < best_ms) { best_ms = ms; best = ft; }
     }
     {

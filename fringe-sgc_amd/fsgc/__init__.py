@@ -237,7 +237,7 @@ class Graph:
         out = []
         for r in reps[: nr.value]:
             out.append({"schedule": SCHEDULE_NAMES[r.schedule], "candidates": r.candidates,
-                        "grid": r.best.grid, "chunk": r.best.chunk, "hub_degree": r.best.hub_degree,
+                        "block": r.best.block, "grid": r.best.grid, "chunk": r.best.chunk, "hub_degree": r.best.hub_degree,
                         "best_ms": r.best_ms, "default_ms": r.default_ms})
         return out
 

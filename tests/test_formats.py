@@ -103,3 +103,18 @@ def test_reproduce_library_matches_reference_families():
     sizes = {f: len(reproduce.library(f)) for f in reproduce.FAMILIES}
     assert sizes == {"vertex": 5, "edge": 6, "wedge": 4, "triangle": 4, "tails": 11, "tri_fringe": 11,
                      "fig3wedge": 11}
+
+
+def test_config_graphs_match_committed_sha256():
+    """The generators still produce the committed benchmark graphs (C1 and C2 here; the
+    larger ones are in the same file and are regenerated by make_graph_sha.py)."""
+    import hashlib
+    import json
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "graph_sha256.json")))
+    for spec in ("er:4096:32768:1", "rmat:20:16:0.57:0.19:0.19:7"):
+        ni, nl = fsgc.generate_csr(spec)
+        h = hashlib.sha256()
+        h.update(ni.astype("<i4").tobytes())
+        h.update(nl.astype("<i4").tobytes())
+        assert h.hexdigest() == ref[spec]["sha256"], spec
+        assert len(nl) == ref[spec]["directed_edges"]
