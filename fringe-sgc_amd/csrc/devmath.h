@@ -284,14 +284,46 @@ __device__ __noinline__ u128 fc3_collapsed(int fx, int fy, int fz, int x, int y,
   return count;
 }
 
+// fc3_collapsed when f_Y, f_Z <= 1, written out (regions non-negative). With
+// b_c = C(n_X - c, f_X):
+//   f_Y = f_Z = 0:  b_0
+//   f_Y = 1:        b_0 (y + yz) + b_1 (xy + xyz)                  (f_Z = 1 mirrors it)
+//   f_Y = f_Z = 1:  b_0 [y (z + yz) + yz (z + yz - 1)]
+//                 + b_1 [(xy + xyz)(z + yz) + (xz + xyz)(y + yz)]
+//                 + b_2 [xy (xz + xyz) + xyz (xz + xyz - 1)]
+// (the (k1, k2) terms of the collapsed sum). The bracketed factors stay below 2^40.
+__device__ __noinline__ u128 fc3_small(int fx, int fy, int fz, int x, int y, int z, int xy, int xz, int yz, int xyz,
+                                          uint32_t& fl)
+{
+  const int nx = x + xy + xz + xyz;
+  const u128 b0 = binom(nx, fx, fl);
+  if ((fy | fz) == 0) return b0;
+  const u128 b1 = binom(nx - 1, fx, fl);
+  if (fz == 0) return addx(mulx(b0, (u128)(ull)(y + yz), fl), mulx(b1, (u128)(ull)(xy + xyz), fl), fl);
+  if (fy == 0) return addx(mulx(b0, (u128)(ull)(z + yz), fl), mulx(b1, (u128)(ull)(xz + xyz), fl), fl);
+  const u128 b2 = binom(nx - 2, fx, fl);
+  const long long zyz = z + yz, xzxyz = xz + xyz;
+  const ull t0 = (ull)((long long)y * zyz + (long long)yz * (zyz - 1));
+  const ull t1 = (ull)((long long)(xy + xyz) * zyz + xzxyz * (long long)(y + yz));
+  const ull t2 = (ull)((long long)xy * xzxyz + (long long)xyz * (xzxyz - 1));
+  return addx(addx(mulx(b0, (u128)t0, fl), mulx(b1, (u128)t1, fl), fl), mulx(b2, (u128)t2, fl), fl);
+}
+
+__device__ __forceinline__ u128 fc3_xyz(int fx, int fy, int fz, int x, int y, int z, int xy, int xz, int yz, int xyz,
+                                        uint32_t& fl)
+{
+  if (fy <= 1 && fz <= 1 && (x | y | z | xy | xz | yz | xyz) >= 0) return fc3_small(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+  return fc3_collapsed(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+}
+
 __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, uint32_t& fl)
 {
   const int fu = fr[iu], fv = fr[iv], fw = fr[iw];
   if (fu >= fv && fu >= fw)
-    return fc3_collapsed(fu, fv, fw, venn[iu], venn[iv], venn[iw], venn[iuv], venn[iuw], venn[ivw], venn[iuvw], fl);
+    return fc3_xyz(fu, fv, fw, venn[iu], venn[iv], venn[iw], venn[iuv], venn[iuw], venn[ivw], venn[iuvw], fl);
   if (fv >= fw)
-    return fc3_collapsed(fv, fu, fw, venn[iv], venn[iu], venn[iw], venn[iuv], venn[ivw], venn[iuw], venn[iuvw], fl);
-  return fc3_collapsed(fw, fu, fv, venn[iw], venn[iu], venn[iv], venn[iuw], venn[ivw], venn[iuv], venn[iuvw], fl);
+    return fc3_xyz(fv, fu, fw, venn[iv], venn[iu], venn[iw], venn[iuv], venn[ivw], venn[iuw], venn[iuvw], fl);
+  return fc3_xyz(fw, fu, fv, venn[iw], venn[iu], venn[iv], venn[iuw], venn[ivw], venn[iuv], venn[iuvw], fl);
 }
 
 __device__ __forceinline__ bool fc3_is_singleton(const int* fr) { return (fr[iuv] | fr[iuw] | fr[ivw] | fr[iuvw]) == 0; }

@@ -12,8 +12,10 @@
 //     bit), so every wedge with endpoint w reads c_vw and v ~ w with one load;
 //   * for each centre u of w, c3(u, v, w) for all v is the number of x in T(u,w)
 //     with v in T(u,x): a scatter over the precomputed lists into LDS counters E
-//     indexed by position in N(u);
-//   * a wave then walks N(u) past w and evaluates every wedge job per lane.
+//     indexed by position in N(u), one window of kECap positions at a time (hub
+//     centres take several windows; each x's list is cut to the window by two
+//     binary searches);
+//   * a wave then walks the window and evaluates every wedge job per lane.
 // The sum over wedges is the reference's, with the same per-wedge filters, formulas
 // and counted matches; only the order of evaluation differs.
 #include <hip/hip_runtime.h>
@@ -221,15 +223,13 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
                                                            const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
                                                            int nitems, int* __restrict__ head,
-                                                           unsigned* __restrict__ dbuf, long long dstride,
-                                                           unsigned* __restrict__ ebuf, long long estride)
+                                                           unsigned* __restrict__ dbuf, long long dstride)
 {
   __shared__ unsigned s_e[kPivWaves][kECap];
   __shared__ int s_item, s_next;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
   unsigned* __restrict__ D = dbuf + (long long)blockIdx.x * dstride;
-  unsigned* __restrict__ Eg = ebuf + ((long long)blockIdx.x * kPivWaves + wave) * estride;
   const G g{idx, adj};
   LaneAcc acc[NJ];
   // algorithmic global-memory bytes of this schedule: every logical load, store or
@@ -271,67 +271,86 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       if (jobs[0].smod > 1) in_slice = (u % jobs[0].smod) == jobs[0].srem;
       if (!in_slice) continue;
       if (lane == 0) bytes += 16 + 4 * (32 - __clz(du + 1));
-      const bool lds = du <= kECap;
-      unsigned* __restrict__ E = lds ? s_e[wave] : Eg;
+      unsigned* __restrict__ E = s_e[wave];
       const bool c3on = need_c3 && tuw > 0;
-      if (c3on) {
-        for (int q = rq + 1 + lane; q < du; q += kWave) E[q] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // scatter: for x in T(u, w), for y in T(u, x) with y > rq: E[y]++
-        const long long tb = tl_off[ub + rq], te = tl_off[ub + rq + 1];
-        for (long long t0 = tb; t0 < te; t0 += kWave) {
-          const long long t = t0 + lane;
-          int xq = -1;
-          if (t < te) xq = tl[t];
-          // each lane walks its own list; long lists are shared by the wave
-          long long lb = 0, le = 0;
-          if (xq >= 0) {
-            lb = tl_off[ub + xq];
-            le = tl_off[ub + xq + 1];
-            long long lo = lb, hi = le;  // first entry > rq
-            while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] <= rq) lo = mid + 1; else hi = mid; }
-            lb = lo;
+      const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
+      // the walk past w in windows of kECap positions, each with its c3 counters in LDS
+      for (int q_lo = rq + 1; q_lo < du; q_lo += kECap) {
+        const int q_hi = min(du, q_lo + kECap);
+        const bool last = q_hi == du;
+        if (c3on) {
+          for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          // scatter: for x in T(u, w), for y in T(u, x) with q_lo <= y < q_hi: E[y - q_lo]++
+          for (long long t0 = tb; t0 < te; t0 += kWave) {
+            const long long t = t0 + lane;
+            int xq = -1;
+            if (t < te) xq = tl[t];
+            // each lane walks its own list; long lists are shared by the wave
+            long long lb = 0, le = 0;
+            if (xq >= 0) {
+              const long long b0 = tl_off[ub + xq], e0 = tl_off[ub + xq + 1];
+              long long lo = b0, hi = e0;  // first entry >= q_lo
+              while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
+              lb = lo;
+              le = e0;
+              if (!last) {  // first entry >= q_hi
+                hi = e0;
+                while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
+                le = lo;
+              }
+              bytes += 20 + 8 * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) + (le - lb) * 4;
+            }
+            const bool big = (le - lb) > 64;
+            if (!big) {
+              long long k = lb;
+              for (; k + 4 <= le; k += 4) {
+                const int a0 = tl[k], a1 = tl[k + 1], a2 = tl[k + 2], a3 = tl[k + 3];
+                atomicAdd(&E[a0 - q_lo], 1u);
+                atomicAdd(&E[a1 - q_lo], 1u);
+                atomicAdd(&E[a2 - q_lo], 1u);
+                atomicAdd(&E[a3 - q_lo], 1u);
+              }
+              for (; k < le; k++) atomicAdd(&E[tl[k] - q_lo], 1u);
+            }
+            ull bm = __ballot(big);
+            while (bm) {
+              const int who = __builtin_ctzll(bm);
+              bm &= bm - 1;
+              const long long B = bcast(lb, who), Ee = bcast(le, who);
+              for (long long k = B + lane; k < Ee; k += kWave) atomicAdd(&E[tl[k] - q_lo], 1u);
+            }
           }
-          if (xq >= 0) bytes += 20 + 8 * (64 - __clzll(le - lb + 1)) + (le - lb) * (lds ? 4 : 12);
-          const bool big = (le - lb) > 64;
-          if (!big) {
-            for (long long k = lb; k < le; k++) atomicAdd(&E[tl[k]], 1u);
-          }
-          ull bm = __ballot(big);
-          while (bm) {
-            const int who = __builtin_ctzll(bm);
-            bm &= bm - 1;
-            const long long B = bcast(lb, who), Ee = bcast(le, who);
-            for (long long k = B + lane; k < Ee; k += kWave) atomicAdd(&E[tl[k]], 1u);
-          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-      // walk N(u) past w
-      for (int q0 = rq + 1; q0 < du; q0 += kWave) {
-        const int q = q0 + lane;
-        if (q < du) {
-          const int v = adj[ub + q];
-          const unsigned dv_ = D[v];
-          const bool tri = dv_ & 1u;
-          const int cvw = (int)(dv_ >> 1);
-          const int tuv = tsup[ub + q];
-          const int c3 = c3on ? (int)E[q] : 0;
-          const int dv = g.deg(v);
-          bytes += 20 + ((c3on && !lds) ? 4 : 0);
+        // walk the window
+        for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
+          const int q = q0 + lane;
+          if (q < q_hi) {
+            const int v = adj[ub + q];
+            const unsigned dv_ = D[v];
+            const bool tri = dv_ & 1u;
+            const int cvw = (int)(dv_ >> 1);
+            const int tuv = tsup[ub + q];
+            const int c3 = c3on ? (int)E[q - q_lo] : 0;
+            const int dv = g.deg(v);
+            bytes += 20;
 #pragma unroll
-          for (int j = 0; j < NJ; j++) {
-            if (j < njobs) {
-              u128 val;
-              if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
-                acc[j].add(val);
-                acc[j].matches++;
+            for (int j = 0; j < NJ; j++) {
+              if (j < njobs) {
+                u128 val;
+                if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
+                  acc[j].add(val);
+                  acc[j].matches++;
+                }
               }
             }
           }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -449,13 +468,12 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   return 0;
 }
 
-// Grid of the pivot launch (before clamping to the number of items) and the device
-// scratch it takes: one dense D array of n counters per workgroup and one spill array
-// of maxdeg counters per wave for centres whose list does not fit in LDS.
 // waves per workgroup of the pivot launch: Tune::block of 512 or 1024 threads selects
 // the 8- and 16-wave shapes, anything else the 4-wave one
 static int pivot_waves(const Tune& t) { return t.block >= 1024 ? 16 : (t.block >= 512 ? 8 : 4); }
 
+// Grid of the pivot launch (before clamping to the number of items); its device
+// scratch is one dense D array of n counters per workgroup.
 static int pivot_grid(const Tune& t)
 {
   int dev = 0, cus = 256;
@@ -468,8 +486,8 @@ static int pivot_grid(const Tune& t)
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
   const size_t grid = (size_t)pivot_grid(t);
-  const size_t dstride = ((size_t)g.n + 63) / 64 * 64, estride = ((size_t)g.maxdeg + 63) / 64 * 64;
-  return sizeof(unsigned) * grid * (dstride + estride * pivot_waves(t)) + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
+  const size_t dstride = ((size_t)g.n + 63) / 64 * 64;
+  return sizeof(unsigned) * grid * dstride + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
 }
 
 // Prepares one pivot-schedule launch of up to kMaxPivotJobs wedge-class jobs.
@@ -510,17 +528,14 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
 
   const int grid = std::max(1, std::min<int>(pivot_grid(t), (int)sorted.size()));
   const long long dstride = ((long long)g->n + 63) / 64 * 64;
-  const long long estride = ((long long)g->maxdeg + 63) / 64 * 64;
   rc = plan_accumulators(lp, nj, stream, err);
   if (rc) return rc;
   DevScratch& sc = *lp->scratch;
   unsigned* dbuf = nullptr;
-  unsigned* ebuf = nullptr;
   JobDesc* d_jobs = nullptr;
   dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
   const int waves = pivot_waves(t);
-  HIP_TRY(sc.get(&ebuf, sizeof(unsigned) * estride * grid * waves));
   int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
@@ -549,7 +564,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP)                                                                   \
   hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
-                     tsup, tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride, ebuf, estride)
+                     tsup, tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride)
 #define FSGC_PIVOT_SHAPE(NJ, FULL)                                          \
   do {                                                                      \
     if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048);                 \
