@@ -9,7 +9,8 @@ of that motif over the whole graph (fsgc_count; the per-graph edge index is buil
 once, before the timed region, and reported as index_build_ms).
 
 Metric: core matches per second (core embeddings whose fringe formula is evaluated).
-Also reported: the roofline of the dominant kernel (k_wedge_pivot) from its
+Also reported: the roofline of the dominant kernel (k_wedge_pivot for the default
+workload, k_triangle_rank for --workload c1 / c4) from its
 algorithmic bytes and its HIP-event time, and a CPU baseline: the oracle (the
 reference's core+fringe method restated in C++/OpenMP) on a deterministic slice of
 root vertices, checked bit-exact against the GPU on the same slice. Beside it, the
@@ -37,18 +38,21 @@ WORKLOADS = {
         "motif": (8, [(0, 1), (1, 2), (1, 3), (1, 4), (1, 5), (0, 6), (2, 7)]),
         "name": "R-MAT s20 ef16 (0.57,0.19,0.19) seed 7; wedge core + 3 centre / 1+1 endpoint fringes",
         "cpu_slice": (4096,),
+        "kernel": "k_wedge_pivot",
     },
     "c1": {
         "spec": "er:4096:32768:1",
         "motif": (5, [(0, 1), (1, 2), (0, 2), (0, 3), (0, 4)]),
         "name": "ER G(4096, 32768) seed 1; triangle + 2 fringes on one core vertex",
         "cpu_slice": (1,),
+        "kernel": "k_triangle_rank",
     },
     "c4": {
         "spec": "chunglu:4000000:2.1:20:100000:19",
         "motif": (9, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (0, 4), (0, 5), (0, 6), (1, 7), (1, 8)]),
         "name": "Chung-Lu n=4e6 gamma 2.1 mean 20 max 1e5 seed 19; 4-clique + 3/2 fringes",
         "cpu_slice": (4096,),
+        "kernel": "k_triangle_rank",
     },
 }
 
@@ -327,8 +331,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-        "traffic": load_traffic("k_wedge_pivot", args.workload),
-        "kernel": "k_wedge_pivot" if r.motif_class == fsgc.CLASS_NAMES.index("wedge") else "k_count",
+        "traffic": load_traffic(W["kernel"], args.workload),
+        "kernel": W["kernel"],
         "kernel_ms": round(kernel_ms, 3),
         "alg_bytes_per_launch": r.alg_bytes,
     }

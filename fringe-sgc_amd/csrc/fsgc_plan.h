@@ -20,7 +20,7 @@ HostGraph motif_graph(const MatchingOrder& mo);
 
 // Launch parameters of the persistent counting kernel.
 struct Tune {
-  int block = 256;        // threads per workgroup (multiple of 64)
+  int block = 0;          // threads per workgroup for kernels with several shapes (0: the kernel's default)
   int grid = 0;           // workgroups (0: 8 per CU)
   int chunk = 256;        // roots (CSR positions or vertices) per work item
   int hub_degree = 4096;  // roots of degree above this are split into single-root items
@@ -46,8 +46,10 @@ struct DeviceGraph {
   int32_t* idx = nullptr;   // n + 1
   int32_t* adj = nullptr;   // m
   std::vector<int32_t> h_idx;  // host copy of nindex (used to cut work items)
-  // derived once per graph for the pivot schedule
-  bool derived_ready = false;
+  // derived once per graph: supports and pivot costs, then (pivot and rank
+  // schedules) the common-neighbour lists
+  bool support_ready = false;
+  bool derived_ready = false;   // lists built too
   int32_t* tsup = nullptr;      // m: |N(u) & N(x)| per directed edge
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
@@ -100,7 +102,7 @@ void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out);
 int launch_run(const LaunchPlan& lp, void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
 // pivot.hip
-int device_derived_build(DeviceGraph* g, void* stream, double* ms, std::string* err);
+int device_derived_build(DeviceGraph* g, void* stream, double* ms, std::string* err, bool lists);
 void device_derived_free(DeviceGraph* g);
 constexpr int kMaxPivotJobs = 8;
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t);

@@ -9,7 +9,8 @@
 //     are the per-edge common-neighbour lists T(u,x) as positions in N(u);
 //   * a workgroup takes a pivot w (the smaller endpoint) and scatters the 2-hop
 //     counts c(w, y) for y > w into a dense per-workgroup array D (plus an adjacency
-//     bit), so every wedge with endpoint w reads c_vw and v ~ w with one load;
+//     bit, and the item's epoch so that D needs no clearing pass), so every wedge
+//     with endpoint w reads c_vw and v ~ w with one load;
 //   * for each centre u of w, c3(u, v, w) for all v is the number of x in T(u,w)
 //     with v in T(u,x): a scatter over the precomputed lists into LDS counters E
 //     indexed by position in N(u), one window of kECap positions at a time (hub
@@ -235,6 +236,12 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   // algorithmic global-memory bytes of this schedule: every logical load, store or
   // atomic on a global array counted at its size (atomics twice), no cache credit
   ull bytes = 0;
+  // D entries carry the item's epoch in their top 8 bits: an entry written for an
+  // earlier item reads as zero, so no pass clears D after an item. Every 255 items,
+  // and when the workgroup finishes, D is cleared whole (sequential stores), which
+  // also leaves it zero for the next launch.
+  unsigned epoch = 0;
+  const long long dn = dstride;
 
   for (;;) {
     if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; }
@@ -243,15 +250,27 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     if (it >= nitems) break;
     const PivotItem I = items[it];
     const int w = I.w, wb = g.beg(w), we = g.end(w), dw = we - wb;
+    if (++epoch == 256) {
+      for (long long i = threadIdx.x; i < dn; i += kPivWaves * kWave) D[i] = 0u;
+      if (threadIdx.x == 0) bytes += 4 * dn;
+      __syncthreads();
+      epoch = 1;
+    }
+    const unsigned e0 = epoch << 24;
 
     // Phase A: D[y] += 2 for every path w - x - y with y > w; D[x] |= 1 for x in N(w).
     for (int i = wb + wave; i < we; i += kPivWaves) {
       const int x = adj[i];
       const int xb = g.beg(x), xe = g.end(x);
-      if (lane == 0 && x > w) atomicOr(&D[x], 1u);
+      if (lane == 0 && x > w) { atomicMax(&D[x], e0); atomicOr(&D[x], 1u); }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 8 : 0);
-      for (int k = s + lane; k < xe; k += kWave) { atomicAdd(&D[adj[k]], 2u); bytes += 12; }
+      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 16 : 0);
+      for (int k = s + lane; k < xe; k += kWave) {
+        const int y = adj[k];
+        atomicMax(&D[y], e0);
+        atomicAdd(&D[y], 2u);
+        bytes += 20;
+      }
     }
     __syncthreads();
 
@@ -330,7 +349,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           const int q = q0 + lane;
           if (q < q_hi) {
             const int v = adj[ub + q];
-            const unsigned dv_ = D[v];
+            const unsigned draw = D[v];
+            const unsigned dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
             const bool tri = dv_ & 1u;
             const int cvw = (int)(dv_ >> 1);
             const int tuv = tsup[ub + q];
@@ -355,17 +375,11 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-
-    // Phase C: clear D
-    for (int i = wb + wave; i < we; i += kPivWaves) {
-      const int x = adj[i];
-      const int xb = g.beg(x), xe = g.end(x);
-      if (lane == 0 && x > w) D[x] = 0u;
-      const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 4 : 0);
-      for (int k = s + lane; k < xe; k += kWave) { D[adj[k]] = 0u; bytes += 8; }
-    }
-    __syncthreads();
+  }
+  // leave D zero for the next launch
+  if (epoch > 0) {
+    for (long long i = threadIdx.x; i < dn; i += kPivWaves * kWave) D[i] = 0u;
+    if (threadIdx.x == 0) bytes += 4 * dn;
   }
 #pragma unroll
   for (int j = 0; j < NJ; j++)
@@ -407,24 +421,25 @@ void device_derived_free(DeviceGraph* g)
   g->tl = nullptr;
   g->piv_cost = nullptr;
   g->derived_ready = false;
+  g->support_ready = false;
 }
 
-// Builds the per-edge supports and common-neighbour lists (once per graph).
-int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string* err)
+// Builds the per-edge supports and pivot costs, and (lists = true) the per-edge
+// common-neighbour lists, each once per graph. The lists hold 6 entries per triangle;
+// schedules that need only the supports (the edge pass) never allocate them.
+int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string* err, bool lists)
 {
   // one build per graph, also when several host threads count on it at once
   static std::mutex mu;
   std::lock_guard<std::mutex> lk(mu);
-  if (g->derived_ready) { if (ms) *ms = 0; return 0; }
+  if (ms) *ms = 0;
+  if (g->derived_ready || (!lists && g->support_ready)) return 0;
   hipStream_t stream = (hipStream_t)stream_v;
   const int n = g->n;
   const long long m = g->m;
   EventPair ev;
   HIP_TRY(ev.create());
   HIP_TRY(hipEventRecord(ev.a, stream));
-  HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
-  HIP_TRY(hipMalloc((void**)&g->tl_off, sizeof(long long) * (m + 1)));
-  HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
   DevScratch sc(stream);
   int* d_head = nullptr;
   HIP_TRY(sc.get(&d_head, sizeof(int) * 2));
@@ -433,44 +448,58 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int grid = std::max(1, std::min(cus * 8, (n + 3) / 4));
-  hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
-                     (const long long*)nullptr, (int*)nullptr, d_head);
-  HIP_TRY(hipGetLastError());
-  // tl_off = exclusive prefix sum of tsup (64-bit)
-  HIP_TRY(hipMemsetAsync(g->tl_off, 0, sizeof(long long), stream));
-  size_t tmp_bytes = 0;
-  hipcub::TransformInputIterator<long long, dev::Widen, const int*> in(g->tsup, dev::Widen());
-  HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
-  void* tmp = nullptr;
-  HIP_TRY(sc.get(&tmp, tmp_bytes + 16));
-  HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
-  long long total = 0;
-  HIP_TRY(hipMemcpyAsync(&total, g->tl_off + m, sizeof(long long), hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  g->tl_total = total;
-  HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
-  HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
-  hipLaunchKernelGGL(dev::k_common<true>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
-                     (const long long*)g->tl_off, g->tl, d_head + 1);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
-  HIP_TRY(hipGetLastError());
+  if (!g->support_ready) {
+    HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
+    HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
+    hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
+                       (const long long*)nullptr, (int*)nullptr, d_head);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
+    HIP_TRY(hipGetLastError());
+    g->h_piv_cost.resize(n);
+    HIP_TRY(hipMemcpyAsync(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    g->derived_bytes = sizeof(int) * (size_t)m + sizeof(long long) * (size_t)n;
+    g->support_ready = true;
+  }
+  if (lists) {
+    // tl_off = exclusive prefix sum of tsup (64-bit)
+    HIP_TRY(hipMalloc((void**)&g->tl_off, sizeof(long long) * (m + 1)));
+    HIP_TRY(hipMemsetAsync(g->tl_off, 0, sizeof(long long), stream));
+    size_t tmp_bytes = 0;
+    hipcub::TransformInputIterator<long long, dev::Widen, const int*> in(g->tsup, dev::Widen());
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
+    void* tmp = nullptr;
+    HIP_TRY(sc.get(&tmp, tmp_bytes + 16));
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, in, g->tl_off + 1, (int)m, stream));
+    long long total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, g->tl_off + m, sizeof(long long), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    g->tl_total = total;
+    HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
+    HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
+    hipLaunchKernelGGL(dev::k_common<true>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
+                       (const long long*)g->tl_off, g->tl, d_head + 1);
+    HIP_TRY(hipGetLastError());
+    g->derived_bytes += sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total;
+    g->derived_ready = true;
+  }
   HIP_TRY(hipEventRecord(ev.b, stream));
-  g->h_piv_cost.resize(n);
-  HIP_TRY(hipMemcpyAsync(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float t = 0;
   HIP_TRY(hipEventElapsedTime(&t, ev.a, ev.b));
   if (ms) *ms = t;
-  g->derived_bytes = sizeof(int) * (size_t)m + sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total +
-                     sizeof(long long) * (size_t)n;
-  g->derived_ready = true;
   return 0;
 }
 
-// waves per workgroup of the pivot launch: Tune::block of 512 or 1024 threads selects
-// the 8- and 16-wave shapes, anything else the 4-wave one
-static int pivot_waves(const Tune& t) { return t.block >= 1024 ? 16 : (t.block >= 512 ? 8 : 4); }
+// waves per workgroup of the pivot launch: Tune::block of 256 or 512 threads selects the
+// 4- and 8-wave shapes; the default is the 16-wave one (1024 threads, four waves per
+// SIMD against two for the 64 KB shapes: 6.4 s against 8.0 s on R-MAT s20)
+static int pivot_waves(const Tune& t)
+{
+  if (t.block <= 0 || t.block >= 1024) return 16;
+  return t.block >= 512 ? 8 : 4;
+}
 
 // Grid of the pivot launch (before clamping to the number of items); its device
 // scratch is one dense D array of n counters per workgroup.
@@ -497,7 +526,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
   if (nj > dev::kMaxPivotJobs) { if (err) *err = "too many jobs for one pivot launch"; return -7; }
-  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err, true);
   if (rc) return rc;
   bool need_c3 = false;
   for (auto& j : jobs) {

@@ -216,7 +216,8 @@ int fsgc_server_add_graph(fsgc_server* s, fsgc_graph* g, int32_t* id)
   if (!s || !g) { fsgc::set_error("null argument"); return -22; }
   std::string err;
   double ms = 0;
-  const int rc = fsgc::device_derived_build(&g->dg, nullptr, &ms, &err);
+  // the supports now; the common-neighbour lists on the first query that needs them
+  const int rc = fsgc::device_derived_build(&g->dg, nullptr, &ms, &err, false);
   if (rc) { fsgc::set_error(err); return rc; }
   std::lock_guard<std::mutex> lk(s->mu);
   s->graphs.push_back(g);

@@ -115,6 +115,9 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
   const G g{idx, adj};
   LaneAcc acc[NJ];
   const int smod = jobs[0].smod, srem = jobs[0].srem;
+  // algorithmic global-memory bytes: every logical load or store on a global array at
+  // its size, no cache credit (LDS traffic not counted)
+  ull bytes = 0;
   // LDS holds whatever the previous kernel left: clear this wave's marks once
   for (int i = lane; i < kMarkCap; i += kWave) s_mark[wave][i] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -129,13 +132,17 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
     const int x = I.x, xb = g.beg(x), dx = g.deg(x);
     const bool lds = dx <= kMarkCap;
     unsigned char* __restrict__ M = lds ? s_mark[wave] : Mg;
+    if (lane == 0) bytes += 16 + 8;
     for (int py = I.p0; py < I.p1; py++) {
       const int y = adj[xb + py];
       const int dy = g.deg(y);
+      if (lane == 0) bytes += 12;
       if (!rank_less(dx, x, dy, y)) continue;
       const long long yb = tl_off[xb + py], ye = tl_off[xb + py + 1];
+      if (lane == 0) bytes += 16;
       if (ye - yb < 1) continue;
       const int txy = tsup[xb + py];
+      if (lane == 0) bytes += 4 + (need_c3 ? (ye - yb) * (lds ? 8 : 10) : 0);
       // mark T(x, y)
       if (need_c3) {
         for (long long k = yb + lane; k < ye; k += kWave) M[tl[k]] = 1;
@@ -149,11 +156,13 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
           const int pz = tl[k];
           const int z = adj[xb + pz];
           const int dz = g.deg(z);
+          bytes += 16;
           if (rank_less(dy, y, dz, z)) {
             const int txz = tsup[xb + pz];
             const int yb2 = g.beg(y);
             const int pyz = lbound(adj, yb2, g.end(y), z);
             const int tyz = tsup[pyz];
+            bytes += 16 + 4 * (32 - __clz(dy + 1));
             // reference roles by id: u > v > w
             int a = x, da = dx, b = y, db = dy, c = z, dc = dz;
             int tab = txy, tac = txz, tbc = tyz;
@@ -169,6 +178,7 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
             if (need_c3) {
               const long long zb = tl_off[xb + pz], ze = tl_off[xb + pz + 1];
               const int wpos = need_lt ? lbound(adj, xb, xb + dx, c) - xb : 0;  // positions below w = c
+              bytes += 16 + (need_lt ? 4 * (32 - __clz(dx + 1)) : 0) + (ze - zb) * (lds ? 4 : 5);
               for (long long q = zb; q < ze; q++) {
                 const int pq = tl[q];
                 if (M[pq]) {
@@ -206,6 +216,9 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
 #pragma unroll
   for (int j = 0; j < NJ; j++)
     if (j < njobs) flush(acc[j], jobs[j].out);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bytes += __shfl_xor(bytes, o, kWave);
+  if (lane == 0) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
 // One sweep over the directed CSR entries evaluating every edge-core motif of the
@@ -223,14 +236,17 @@ __global__ __launch_bounds__(256) void k_edge_pass(const int* __restrict__ idx, 
   const G g{idx, adj};
   LaneAcc acc[NJ];
   const int smod = jobs[0].smod, srem = jobs[0].srem;
+  ull bytes = 0;  // algorithmic global-memory bytes, as in the rank kernel
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(head, 1);
     it = bcast(it, 0);
     if (it >= nitems) break;
     const WorkItem I = items[it];
+    if (lane == 0) bytes += sizeof(WorkItem);
     for (long long j = I.begin + lane; j < I.end; j += kWave) {
       const int u = owner_of(idx, I.u0, I.u1, j);
+      bytes += 4 * (32 - __clz(I.u1 - I.u0 + 1)) + 28;
       if (smod > 1 && (u % smod) != srem) continue;
       const int v = adj[j];
       const int du = g.deg(u), dv = g.deg(v);
@@ -255,6 +271,9 @@ __global__ __launch_bounds__(256) void k_edge_pass(const int* __restrict__ idx, 
 #pragma unroll
   for (int k = 0; k < NJ; k++)
     if (k < njobs) flush(acc[k], jobs[k].out);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bytes += __shfl_xor(bytes, o, kWave);
+  if (lane == 0) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
 }  // namespace dev
@@ -307,7 +326,7 @@ int plan_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, voi
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
   if (nj > dev::kMaxTriJobs) { if (err) *err = "too many jobs for one edge pass"; return -7; }
-  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err, false);
   if (rc) return rc;
   for (auto& j : jobs) { j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m; }
   std::vector<WorkItem> items;
@@ -379,7 +398,7 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
       tails = j.p[3];
     }
   }
-  int rc = device_derived_build(g, stream_v, &lp->index_ms, err);
+  int rc = device_derived_build(g, stream_v, &lp->index_ms, err, true);
   if (rc) return rc;
   bool need_c3 = false, need_lt = false, full = false;
   for (auto& j : jobs) {

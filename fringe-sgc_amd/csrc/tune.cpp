@@ -23,14 +23,14 @@ std::vector<Cand> candidates(int sched, int cus)
   std::vector<Cand> c;
   std::vector<int> grids, chunks = {64, 256, 1024}, hubs = {4096}, blocks = {256};
   switch (sched) {
-    case fsgc::kPivot: grids = {cus * 2, cus * 4, cus * 8}; blocks = {256, 1024}; break;
+    case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 256}; break;
     case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; break;
     case fsgc::kEdgePass: grids = {cus * 4, cus * 8, cus * 16}; break;
     default: grids = {cus * 4, cus * 8, cus * 16}; hubs = {1024, 4096}; break;
   }
   for (int b : blocks)
     for (int g : grids) {
-      if (b > 256 && g > cus * 4) continue;  // one 1024-thread workgroup per CU fits; more only queue
+      if (b > 256 && g > cus * 2) continue;  // one 1024-thread workgroup per CU fits; more only queue
       for (int ch : chunks)
         for (int h : hubs) c.push_back(Cand{b, g, ch, h});
     }
@@ -79,7 +79,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     const fsgc::Tune d;
     fsgc_tune def;
     std::memset(&def, 0, sizeof(def));
-    def.block = d.block;
+    def.block = s == fsgc::kPivot ? 1024 : 256;
     def.grid = s == fsgc::kPivot || s == fsgc::kRank ? cus * 4 : cus * 8;
     def.chunk = d.chunk;
     def.hub_degree = d.hub_degree;

@@ -67,7 +67,7 @@ typedef struct fsgc_result {
 /* Launch shape. A zero field takes the value fsgc_autotune picked for the graph and
    the motif's schedule, or the default when the graph was not tuned. */
 typedef struct fsgc_tune {
-  int32_t block;       /* threads per workgroup; 0 = default (256) */
+  int32_t block;       /* threads per workgroup where a kernel has several shapes (pivot: 256, 512, 1024); 0 = default */
   int32_t grid;        /* persistent workgroups; 0 = tuned or default (4-8 per CU) */
   int32_t chunk;       /* roots per work item; 0 = tuned or default */
   int32_t hub_degree;  /* roots above this degree get items of their own; 0 = tuned or default */
@@ -159,7 +159,8 @@ typedef struct fsgc_server_stats {
   int32_t slots, peak_concurrent;     /* launches in flight at once (max seen) */
 } fsgc_server_stats;
 int fsgc_server_create(const fsgc_server_config* cfg, fsgc_server** out);
-/* registers a graph (builds its per-edge index now); g must outlive the server */
+/* registers a graph (builds its per-edge supports now, the common-neighbour lists
+   on the first query that needs them); g must outlive the server */
 int fsgc_server_add_graph(fsgc_server* s, fsgc_graph* g, int32_t* id);
 int fsgc_server_submit(fsgc_server* s, int32_t graph, const uint8_t mo[FSGC_MO_BYTES], uint64_t* ticket);
 /* blocks until the query is done, fills out and forgets the ticket */

@@ -87,11 +87,12 @@ def test_tuning_invariance(gpu):
     g = gpu.Graph.from_csr(ni, nl)
     mos = gpu.random_motifs(3, 16) + [gpu.preprocess(*C2_MOTIF)[0], gpu.preprocess(*C4_MOTIF)[0]]
     base = [g.count(mo) for mo in mos]
-    for chunk, grid, hub in ((1, 7, 1), (64, 0, 16), (4096, 33, 100000)):
-        t = gpu.Tune(chunk=chunk, grid=grid, hub_degree=hub)
+    # block selects the pivot kernel's 4-, 8- or 16-wave workgroup shape
+    for chunk, grid, hub, block in ((1, 7, 1, 256), (64, 0, 16, 512), (4096, 33, 100000, 1024), (256, 0, 4096, 0)):
+        t = gpu.Tune(chunk=chunk, grid=grid, hub_degree=hub, block=block)
         for mo, b in zip(mos, base):
             r = g.count(mo, t)
-            assert (r.exact, r.sg, r.core_matches) == (b.exact, b.sg, b.core_matches), (chunk, grid, hub)
+            assert (r.exact, r.sg, r.core_matches) == (b.exact, b.sg, b.core_matches), (chunk, grid, hub, block)
 
 
 def test_random_motifs_vs_oracle(gpu):
