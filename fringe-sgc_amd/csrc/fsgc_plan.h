@@ -51,6 +51,7 @@ struct DeviceGraph {
   bool support_ready = false;
   bool derived_ready = false;   // lists built too
   int32_t* tsup = nullptr;      // m: |N(u) & N(x)| per directed edge
+  int32_t* adjdeg = nullptr;    // m: degree of adj[e] (read in CSR order instead of through idx)
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
   long long tl_total = 0;
@@ -83,7 +84,10 @@ struct LaunchPlan {
   size_t acc_bytes = 0;      // bytes zeroed per run (multiple of 8)
   Accum* h_acc = nullptr;    // pinned host copy of the accumulators
   double index_ms = 0;       // per-graph index build spent while planning
-  std::function<void(void* stream)> kernel;
+  int nitems = 0;            // work items of the launch
+  // launches the kernel over items [first, first + count) with work-queue head `head`
+  std::function<void(void* stream, int first, int count, int* head)> kernel;
+  int* head = nullptr;       // work-queue heads (32 slots), zeroed with the accumulators
   std::shared_ptr<DevScratch> scratch;
   LaunchPlan() = default;
   LaunchPlan(const LaunchPlan&) = delete;
@@ -98,7 +102,9 @@ int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err);
 int launch_enqueue(const LaunchPlan& lp, void* stream, void* ev_begin, void* ev_end, std::string* err);
 // results of a finished run
 void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out);
-// enqueue, wait, collect
+// enqueue, wait, collect. With FSGC_SEGMENT_ITEMS=k in the environment the items run
+// in launches of at most k items each (at most 32), one progress line per launch
+// with FSGC_LOG; the result is the same.
 int launch_run(const LaunchPlan& lp, void* stream, std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 
 // pivot.hip

@@ -586,6 +586,7 @@ LaunchPlan& LaunchPlan::operator=(LaunchPlan&& o) noexcept
   if (this != &o) {
     if (h_acc) (void)hipHostFree(h_acc);
     nj = o.nj; d_acc = o.d_acc; acc_bytes = o.acc_bytes; h_acc = o.h_acc; index_ms = o.index_ms;
+    nitems = o.nitems; head = o.head;
     kernel = std::move(o.kernel);
     scratch = std::move(o.scratch);
     o.h_acc = nullptr;
@@ -605,6 +606,7 @@ int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err)
   lp->nj = nj;
   lp->acc_bytes = sizeof(Accum) * nj + 256;  // the work-queue head lives past the accumulators
   HIP_TRY(lp->scratch->get(&lp->d_acc, lp->acc_bytes));
+  lp->head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   HIP_TRY(hipHostMalloc((void**)&lp->h_acc, sizeof(Accum) * std::max(1, nj), hipHostMallocDefault));
   return 0;
 }
@@ -616,8 +618,8 @@ int launch_enqueue(const LaunchPlan& lp, void* stream_v, void* ev_begin, void* e
   hipLaunchKernelGGL(dev::k_zero64, dim3((words + 255) / 256), dim3(256), 0, stream, (dev::ull*)lp.d_acc, words);
   HIP_TRY(hipGetLastError());
   if (ev_begin) HIP_TRY(hipEventRecord((hipEvent_t)ev_begin, stream));
-  if (lp.kernel) {
-    lp.kernel(stream);
+  if (lp.kernel && lp.nitems > 0) {
+    lp.kernel(stream, 0, lp.nitems, lp.head);
     HIP_TRY(hipGetLastError());
   }
   if (ev_end) HIP_TRY(hipEventRecord((hipEvent_t)ev_end, stream));
@@ -638,17 +640,52 @@ void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out)
   }
 }
 
+static int segment_items()
+{
+  static const int k = [] { const char* e = getenv("FSGC_SEGMENT_ITEMS"); return e ? atoi(e) : 0; }();
+  return k;
+}
+
 int launch_run(const LaunchPlan& lp, void* stream_v, std::vector<JobResult>* out, double* kernel_ms, std::string* err)
 {
   hipStream_t stream = (hipStream_t)stream_v;
-  EventPair ev;
-  HIP_TRY(ev.create());
-  int rc = launch_enqueue(lp, stream_v, ev.a, ev.b, err);
-  if (rc) return rc;
+  const int seg = segment_items();
+  if (seg <= 0 || lp.nitems <= seg || !lp.kernel) {
+    EventPair ev;
+    HIP_TRY(ev.create());
+    int rc = launch_enqueue(lp, stream_v, ev.a, ev.b, err);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    if (kernel_ms) *kernel_ms = ms;
+    launch_collect(lp, out);
+    return 0;
+  }
+  // the same items in consecutive launches, each with its own queue head
+  const int nseg = std::min(32, (lp.nitems + seg - 1) / seg);
+  const int words = (int)(lp.acc_bytes / 8);
+  hipLaunchKernelGGL(dev::k_zero64, dim3((words + 255) / 256), dim3(256), 0, stream, (dev::ull*)lp.d_acc, words);
+  HIP_TRY(hipGetLastError());
+  double total = 0;
+  for (int s = 0; s < nseg; s++) {
+    const int first = (int)((long long)lp.nitems * s / nseg), last = (int)((long long)lp.nitems * (s + 1) / nseg);
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.a, stream));
+    lp.kernel(stream, first, last - first, lp.head + s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev.b, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    total += ms;
+    const char* lg = getenv("FSGC_LOG");
+    if (lg && *lg && *lg != '0') fprintf(stderr, "[fsgc]   segment %d/%d: %d items, %.1f ms\n", s + 1, nseg, last - first, ms);
+  }
+  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, sizeof(Accum) * lp.nj, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (kernel_ms) *kernel_ms = ms;
+  if (kernel_ms) *kernel_ms = total;
   launch_collect(lp, out);
   return 0;
 }
@@ -756,7 +793,6 @@ int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>&
   if (rc) return rc;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
-  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(lp->scratch->get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
@@ -771,10 +807,11 @@ int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>&
   int grid = t.grid > 0 ? t.grid : cus * 8;
   const int nitems = (int)items.size();
   grid = std::max(1, std::min(grid, (nitems + dev::kWavesPerBlock - 1) / dev::kWavesPerBlock));
-  if (nitems > 0)
-    lp->kernel = [=](void* s) {
-      hipLaunchKernelGGL(dev::k_count, dim3(grid), dim3(dev::kBlock), 0, (hipStream_t)s, d_jobs, d_items, nitems, d_head);
-    };
+  lp->nitems = nitems;
+  lp->kernel = [=](void* s, int first, int count, int* head) {
+    hipLaunchKernelGGL(dev::k_count, dim3(grid), dim3(dev::kBlock), 0, (hipStream_t)s, d_jobs, d_items + first, count,
+                       head);
+  };
   return 0;
 }
 

@@ -220,6 +220,7 @@ __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u,
 template <int NJ, bool kFull, int kPivWaves, int kECap>
 __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
                                                            const int* __restrict__ tsup,
+                                                           const int* __restrict__ adjdeg,
                                                            const long long* __restrict__ tl_off,
                                                            const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
@@ -355,8 +356,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
             const int cvw = (int)(dv_ >> 1);
             const int tuv = tsup[ub + q];
             const int c3 = c3on ? (int)E[q - q_lo] : 0;
-            const int dv = g.deg(v);
-            bytes += 20;
+            const int dv = adjdeg[ub + q];
+            bytes += 16;
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
               if (j < njobs) {
@@ -389,6 +390,15 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   if (lane == 0) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
+// degree of the neighbour at every CSR position
+__global__ void k_adjdeg(const int* __restrict__ idx, const int* __restrict__ adj, long long m, int* __restrict__ out)
+{
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (long long)gridDim.x * blockDim.x) {
+    const int x = adj[e];
+    out[e] = idx[x + 1] - idx[x];
+  }
+}
+
 // cost of pivot w: sum of the degrees of its neighbours
 __global__ void k_pivot_cost(const int* __restrict__ idx, const int* __restrict__ adj, int n, long long* __restrict__ cost)
 {
@@ -413,10 +423,12 @@ __global__ void k_pivot_cost(const int* __restrict__ idx, const int* __restrict_
 void device_derived_free(DeviceGraph* g)
 {
   if (g->tsup) (void)hipFree(g->tsup);
+  if (g->adjdeg) (void)hipFree(g->adjdeg);
   if (g->tl_off) (void)hipFree(g->tl_off);
   if (g->tl) (void)hipFree(g->tl);
   if (g->piv_cost) (void)hipFree(g->piv_cost);
   g->tsup = nullptr;
+  g->adjdeg = nullptr;
   g->tl_off = nullptr;
   g->tl = nullptr;
   g->piv_cost = nullptr;
@@ -451,6 +463,9 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   if (!g->support_ready) {
     HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
     HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
+    HIP_TRY(hipMalloc((void**)&g->adjdeg, sizeof(int) * std::max<long long>(1, m)));
+    hipLaunchKernelGGL(dev::k_adjdeg, dim3(cus * 8), dim3(256), 0, stream, g->idx, g->adj, m, g->adjdeg);
+    HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
                        (const long long*)nullptr, (int*)nullptr, d_head);
     HIP_TRY(hipGetLastError());
@@ -459,7 +474,7 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     g->h_piv_cost.resize(n);
     HIP_TRY(hipMemcpyAsync(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
-    g->derived_bytes = sizeof(int) * (size_t)m + sizeof(long long) * (size_t)n;
+    g->derived_bytes = 2 * sizeof(int) * (size_t)m + sizeof(long long) * (size_t)n;
     g->support_ready = true;
   }
   if (lists) {
@@ -565,7 +580,6 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
   const int waves = pivot_waves(t);
-  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(sc.get(&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
@@ -587,13 +601,15 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const int* idx = g->idx;
   const int* adj = g->adj;
   const int* tsup = g->tsup;
+  const int* adjdeg = g->adjdeg;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
-  lp->kernel = [=](void* sv) {
+  lp->nitems = nitems;
+  lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP)                                                                   \
   hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
-                     tsup, tl_off, tl, d_jobs, nj, c3, d_items, nitems, d_head, dbuf, dstride)
+                     tsup, adjdeg, tl_off, tl, d_jobs, nj, c3, d_items + first, count, head, dbuf, dstride)
 #define FSGC_PIVOT_SHAPE(NJ, FULL)                                          \
   do {                                                                      \
     if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048);                 \

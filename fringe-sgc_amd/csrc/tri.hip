@@ -100,6 +100,7 @@ __device__ __forceinline__ bool eval_triangle(const JobDesc* __restrict__ J, int
 template <int NJ, bool kFull>
 __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
                                                              const int* __restrict__ tsup,
+                                                             const int* __restrict__ adjdeg,
                                                              const long long* __restrict__ tl_off,
                                                              const int* __restrict__ tl,
                                                              const JobDesc* __restrict__ jobs, int njobs, int need_c3,
@@ -135,14 +136,18 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
     if (lane == 0) bytes += 16 + 8;
     for (int py = I.p0; py < I.p1; py++) {
       const int y = adj[xb + py];
-      const int dy = g.deg(y);
-      if (lane == 0) bytes += 12;
+      const int dy = adjdeg[xb + py];
+      if (lane == 0) bytes += 8;
       if (!rank_less(dx, x, dy, y)) continue;
       const long long yb = tl_off[xb + py], ye = tl_off[xb + py + 1];
       if (lane == 0) bytes += 16;
       if (ye - yb < 1) continue;
       const int txy = tsup[xb + py];
-      if (lane == 0) bytes += 4 + (need_c3 ? (ye - yb) * (lds ? 8 : 10) : 0);
+      // T(y, x) lists the same common neighbours as T(x, y), in the same (id) order,
+      // as positions in N(y): entry k of one gives entry k of the other
+      const int yb2 = g.beg(y);
+      const long long yxb = tl_off[yb2 + lbound(adj, yb2, g.end(y), x) - yb2];
+      if (lane == 0) bytes += 4 + 12 + 4 * (32 - __clz(dy + 1)) + (need_c3 ? (ye - yb) * (lds ? 8 : 10) : 0);
       // mark T(x, y)
       if (need_c3) {
         for (long long k = yb + lane; k < ye; k += kWave) M[tl[k]] = 1;
@@ -155,14 +160,12 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
         if (k < ye) {
           const int pz = tl[k];
           const int z = adj[xb + pz];
-          const int dz = g.deg(z);
-          bytes += 16;
+          const int dz = adjdeg[xb + pz];
+          bytes += 12;
           if (rank_less(dy, y, dz, z)) {
             const int txz = tsup[xb + pz];
-            const int yb2 = g.beg(y);
-            const int pyz = lbound(adj, yb2, g.end(y), z);
-            const int tyz = tsup[pyz];
-            bytes += 16 + 4 * (32 - __clz(dy + 1));
+            const int tyz = tsup[yb2 + tl[yxb + (k - yb)]];
+            bytes += 12;
             // reference roles by id: u > v > w
             int a = x, da = dx, b = y, db = dy, c = z, dc = dz;
             int tab = txy, tac = txz, tbc = tyz;
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
                   c3++;
                   if (need_lt && pq < wpos) {
                     c3lt++;
-                    if (tails) wq = addx(wq, ncr(g.deg(adj[xb + pq]) - 3, tails, fl0), fl0);
+                    if (tails) wq = addx(wq, ncr(adjdeg[xb + pq] - 3, tails, fl0), fl0);
                   }
                 }
               }
@@ -228,7 +231,8 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
 // vertex range.
 template <int NJ>
 __global__ __launch_bounds__(256) void k_edge_pass(const int* __restrict__ idx, const int* __restrict__ adj,
-                                                  const int* __restrict__ tsup, const JobDesc* __restrict__ jobs,
+                                                  const int* __restrict__ tsup, const int* __restrict__ adjdeg,
+                                                  const JobDesc* __restrict__ jobs,
                                                   int njobs, const WorkItem* __restrict__ items, int nitems,
                                                   int* __restrict__ head)
 {
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(256) void k_edge_pass(const int* __restrict__ idx, 
       bytes += 4 * (32 - __clz(I.u1 - I.u0 + 1)) + 28;
       if (smod > 1 && (u % smod) != srem) continue;
       const int v = adj[j];
-      const int du = g.deg(u), dv = g.deg(v);
+      const int du = g.deg(u), dv = adjdeg[j];
       const bool first = (j == idx[u]);
       const int c = tsup[j];
 #pragma unroll
@@ -348,7 +352,6 @@ int plan_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, voi
   if (rc) return rc;
   JobDesc* d_jobs = nullptr;
   WorkItem* d_items = nullptr;
-  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(lp->scratch->get(&d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
@@ -361,14 +364,16 @@ int plan_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, voi
   const int* idx = g->idx;
   const int* adj = g->adj;
   const int* tsup = g->tsup;
-  lp->kernel = [=](void* sv) {
+  const int* adjdeg = g->adjdeg;
+  lp->nitems = nitems;
+  lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
     if (nj == 1)
-      hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, d_jobs, nj, d_items,
-                         nitems, d_head);
+      hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, adjdeg, d_jobs, nj,
+                         d_items + first, count, head);
     else
-      hipLaunchKernelGGL((dev::k_edge_pass<dev::kMaxTriJobs>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, d_jobs,
-                         nj, d_items, nitems, d_head);
+      hipLaunchKernelGGL((dev::k_edge_pass<dev::kMaxTriJobs>), dim3(grid), dim3(256), 0, s, idx, adj, tsup, adjdeg,
+                         d_jobs, nj, d_items + first, count, head);
   };
   return 0;
 }
@@ -428,7 +433,6 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
   JobDesc* d_jobs = nullptr;
   dev::TriItem* d_items = nullptr;
   HIP_TRY(lp->scratch->get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
-  int* d_head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(lp->scratch->get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
@@ -443,14 +447,16 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
   const int* idx = g->idx;
   const int* adj = g->adj;
   const int* tsup = g->tsup;
+  const int* adjdeg = g->adjdeg;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   const int c3 = need_c3 ? 1 : 0, lt = need_lt ? 1 : 0, tl_arg = std::max(0, tails);
-  lp->kernel = [=](void* sv) {
+  lp->nitems = nitems;
+  lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
   hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, s, idx, adj, tsup,  \
-                     tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items, nitems, d_head, gmark, mstride)
+                     adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
     } else {
