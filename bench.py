@@ -85,7 +85,7 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True):
+def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True, quick=False):
     """Plain path (one fsgc_count per motif) against the batched path (one
     fsgc_count_batch over all motifs) on the same resident graph: motifs/s of each,
     per-class times, and per-motif equality of every result. `classes` restricts the
@@ -122,15 +122,17 @@ def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True
     same = all((a.exact, a.sg, a.aut, a.ref_compat, a.core_matches, a.flags) ==
                (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches, b.flags) for a, b in zip(plain, batch))
     # the same batch captured once as a HIP graph, then replayed
-    cb = g.capture_batch(mos)
-    t0 = time.perf_counter()
-    cap, cap_ms = cb.run()
-    t_cap = time.perf_counter() - t0
-    cb.free()
-    same_cap = all((a.exact, a.sg, a.core_matches) == (b.exact, b.sg, b.core_matches) for a, b in zip(plain, cap))
-    print(f"# captured batch replay: {t_cap:.3f} s, identical {same_cap}", flush=True)
+    t_cap, same_cap = None, None
+    if not quick:
+        cb = g.capture_batch(mos)
+        t0 = time.perf_counter()
+        cap, cap_ms = cb.run()
+        t_cap = time.perf_counter() - t0
+        cb.free()
+        same_cap = all((a.exact, a.sg, a.core_matches) == (b.exact, b.sg, b.core_matches) for a, b in zip(plain, cap))
+        print(f"# captured batch replay: {t_cap:.3f} s, identical {same_cap}", flush=True)
     # the same per class (one count_batch per class)
-    for c, pc in per_class.items():
+    for c, pc in (per_class.items() if not quick else []):
         sub = [mo for mo, r in zip(mos, plain) if r.class_name == c]
         t0 = time.perf_counter()
         g.count_batch(sub)
@@ -149,7 +151,8 @@ def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True
         "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
         "plain_motifs_per_s": round(len(mos) / t_plain, 4), "batched_motifs_per_s": round(len(mos) / t_batch, 4),
         "speedup": round(t_plain / t_batch, 3), "identical": same,
-        "captured_s": round(t_cap, 3), "captured_speedup": round(t_plain / t_cap, 3), "captured_identical": same_cap,
+        "captured_s": round(t_cap, 3) if t_cap else None,
+        "captured_speedup": round(t_plain / t_cap, 3) if t_cap else None, "captured_identical": same_cap,
         "per_class": per_class,
     }
     print(json.dumps(res) if emit else "# " + json.dumps(res), flush=True)
@@ -268,6 +271,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
     ap.add_argument("--batch", default=None, help="graph spec: compare plain vs batched over the random motifs")
     ap.add_argument("--batch-out", default=None)
+    ap.add_argument("--batch-quick", action="store_true", help="--batch: skip the captured replay and per-class runs")
     ap.add_argument("--classes", default=None, help="comma-separated motif classes for --batch / --serve")
     ap.add_argument("--serve", default=None, help="comma-separated graph specs: served Poisson query stream")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve: offered queries per second")
@@ -277,7 +281,7 @@ def main():
     args = ap.parse_args()
     classes = set(args.classes.split(",")) if args.classes else None
     if args.batch:
-        batch_compare(args.batch, classes=classes, out_path=args.batch_out)
+        batch_compare(args.batch, classes=classes, out_path=args.batch_out, quick=args.batch_quick)
         return
     if args.serve:
         serve_bench(args.serve.split(","), args.rate, args.queries, args.slots, classes=classes,

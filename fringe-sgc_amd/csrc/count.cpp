@@ -140,10 +140,19 @@ namespace {
 // Motifs share a launch only within a tier: a launch runs the kernel variant its most
 // demanding motif needs (the c3 pass, the full fc3 loop), so mixing tiers would make
 // the cheap motifs pay for it.
+// a wedge or triangle job whose fringe vectors have a multi-vertex anchor set (the
+// full fc3 kernel variant)
+bool needs_full(const JobDesc& j)
+{
+  if (j.cls != kWedge && j.cls != kTriangle) return false;
+  bool multi = false;
+  for (int r = 0; r < j.rot; r++) multi = multi || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+  return multi;
+}
+
 int group_tier(const JobDesc& j)
 {
-  bool multi = false;  // a fringe vector with a multi-vertex anchor set (full fc3)
-  for (int r = 0; r < j.rot; r++) multi = multi || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+  const bool multi = needs_full(j);
   if (j.cls == kEdge) return 0;
   if (j.cls == kWedge) return j.kind == kWedgeSpecial ? 0 : (multi ? 2 : 1);
   if (j.cls == kTriangle) return j.kind != kTriGeneral ? 1 : (multi ? 3 : 2);
@@ -172,7 +181,10 @@ std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast)
       for (auto& gr : groups) {
         const auto& j0 = ps[gr.idx[0]].job;
         const int k0 = (j0.cls == kTriangle && j0.kind == kTriTailedClique) ? j0.p[3] : -1;
-        if ((int)gr.idx.size() < kMaxPivotJobs && group_tier(j0) == tier && (key < 0 || k0 < 0 || key == k0)) {
+        // launches of the full fc3 carry at most two motifs: their per-wedge formula
+        // dominates, and the 8-motif variant of that kernel spills to scratch
+        const int cap = needs_full(j0) ? 2 : kMaxPivotJobs;
+        if ((int)gr.idx.size() < cap && group_tier(j0) == tier && (key < 0 || k0 < 0 || key == k0)) {
           gr.idx.push_back(i);
           placed = true;
           break;

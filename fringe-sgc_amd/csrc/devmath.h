@@ -328,6 +328,46 @@ __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, u
 
 __device__ __forceinline__ bool fc3_is_singleton(const int* fr) { return (fr[iuv] | fr[iuw] | fr[ivw] | fr[iuvw]) == 0; }
 
+__device__ __forceinline__ bool venn_nonneg(const int* venn)
+{
+  return (venn[iu] | venn[iv] | venn[iw] | venn[iuv] | venn[iuw] | venn[ivw] | venn[iuvw]) >= 0;
+}
+
+// fc3 with multi-vertex anchor sets, non-negative regions: the reference's outer
+// loops place the uvw, uv, uw and vw fringes (FC:249-276) and, for each placement,
+// the single-vertex fringes are counted in closed or collapsed form on the regions
+// that remain (instead of the four inner loops, FC:277-318).
+static __device__ __noinline__ u128 fc3_peeled(const int* fr, const int* venn, uint32_t& fl)
+{
+  int v[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) v[r] = venn[r];
+  if (v[iuvw] < fr[iuvw]) return 0;
+  const u128 cuvw = binom(v[iuvw], fr[iuvw], fl);
+  v[iuvw] -= fr[iuvw];
+  u128 total = 0;
+  for (int i = 0; i <= min(v[iuvw], fr[iuv]); i++) {
+    if (v[iuv] < fr[iuv] - i) continue;
+    const u128 c1 = mulx(mulx(cuvw, binom(v[iuvw], i, fl), fl), binom(v[iuv], fr[iuv] - i, fl), fl);
+    v[iuvw] -= i; v[iuv] -= fr[iuv] - i;
+    for (int j = 0; j <= min(v[iuvw], fr[iuw]); j++) {
+      if (v[iuw] < fr[iuw] - j) continue;
+      const u128 c2 = mulx(mulx(c1, binom(v[iuvw], j, fl), fl), binom(v[iuw], fr[iuw] - j, fl), fl);
+      v[iuvw] -= j; v[iuw] -= fr[iuw] - j;
+      for (int k = 0; k <= min(v[iuvw], fr[ivw]); k++) {
+        if (v[ivw] < fr[ivw] - k) continue;
+        const u128 c3 = mulx(mulx(c2, binom(v[iuvw], k, fl), fl), binom(v[ivw], fr[ivw] - k, fl), fl);
+        v[iuvw] -= k; v[ivw] -= fr[ivw] - k;
+        total = addx(total, mulx(c3, fc3_singletons(fr, v, fl), fl), fl);
+        v[iuvw] += k; v[ivw] += fr[ivw] - k;
+      }
+      v[iuvw] += j; v[iuw] += fr[iuw] - j;
+    }
+    v[iuvw] += i; v[iuv] += fr[iuv] - i;
+  }
+  return total;
+}
+
 // FC:233-327 (fc3): ways to place the fringes of the 7 anchor sets of a 3-vertex core
 // disjointly into the 7 Venn regions (loop order uvw, uv, uw, vw, u; v and w merged).
 static __device__ __noinline__ u128 fc3(const int* fr, const int* venn, uint32_t& fl)

@@ -163,8 +163,9 @@ struct WedgeEval {
 template <bool kFull>
 __device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t& fl)
 {
+  if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
   if constexpr (kFull) {
-    if (!fc3_is_singleton(fr)) return fc3(fr, venn, fl);
+    if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
   return fc3_singletons(fr, venn, fl);
 }
@@ -266,7 +267,21 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       if (lane == 0 && x > w) { atomicMax(&D[x], e0); atomicOr(&D[x], 1u); }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
       if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 16 : 0);
-      for (int k = s + lane; k < xe; k += kWave) {
+      // four list entries per lane in flight before their atomics
+      int k = s + lane;
+      for (; k + 3 * kWave < xe; k += 4 * kWave) {
+        const int y0 = adj[k], y1 = adj[k + kWave], y2 = adj[k + 2 * kWave], y3 = adj[k + 3 * kWave];
+        atomicMax(&D[y0], e0);
+        atomicMax(&D[y1], e0);
+        atomicMax(&D[y2], e0);
+        atomicMax(&D[y3], e0);
+        atomicAdd(&D[y0], 2u);
+        atomicAdd(&D[y1], 2u);
+        atomicAdd(&D[y2], 2u);
+        atomicAdd(&D[y3], 2u);
+        bytes += 80;
+      }
+      for (; k < xe; k += kWave) {
         const int y = adj[k];
         atomicMax(&D[y], e0);
         atomicAdd(&D[y], 2u);
@@ -345,18 +360,29 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           __builtin_amdgcn_wave_barrier();
         }
-        // walk the window
+        // walk the window; the loads of the next 64 positions are issued before the
+        // current ones are evaluated
+        unsigned nd = 0;
+        int ntuv = 0, ndv = 0;
+        if (q_lo + lane < q_hi) {
+          nd = D[adj[ub + q_lo + lane]];
+          ntuv = tsup[ub + q_lo + lane];
+          ndv = adjdeg[ub + q_lo + lane];
+        }
         for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
           const int q = q0 + lane;
+          const unsigned draw = nd;
+          const int tuv = ntuv, dv = ndv;
+          if (q + kWave < q_hi) {
+            nd = D[adj[ub + q + kWave]];
+            ntuv = tsup[ub + q + kWave];
+            ndv = adjdeg[ub + q + kWave];
+          }
           if (q < q_hi) {
-            const int v = adj[ub + q];
-            const unsigned draw = D[v];
             const unsigned dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
             const bool tri = dv_ & 1u;
             const int cvw = (int)(dv_ >> 1);
-            const int tuv = tsup[ub + q];
             const int c3 = c3on ? (int)E[q - q_lo] : 0;
-            const int dv = adjdeg[ub + q];
             bytes += 16;
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
@@ -618,6 +644,8 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   } while (0)
     if (nj == 1) {
       if (full) FSGC_PIVOT_SHAPE(1, true); else FSGC_PIVOT_SHAPE(1, false);
+    } else if (nj == 2) {
+      if (full) FSGC_PIVOT_SHAPE(2, true); else FSGC_PIVOT_SHAPE(2, false);
     } else {
       if (full) FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, true); else FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, false);
     }

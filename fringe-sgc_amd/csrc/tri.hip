@@ -50,8 +50,9 @@ __device__ __forceinline__ int lbound(const int* __restrict__ a, int l, int r, i
 template <bool kFull>
 __device__ __forceinline__ u128 fc3_tri(const int* fr, const int* venn, uint32_t& fl)
 {
+  if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
   if constexpr (kFull) {
-    if (!fc3_is_singleton(fr)) return fc3(fr, venn, fl);
+    if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
   return fc3_singletons(fr, venn, fl);
 }
@@ -459,6 +460,8 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
                      adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
+    } else if (nj == 2) {
+      if (full) FSGC_TRI_LAUNCH(2, true); else FSGC_TRI_LAUNCH(2, false);
     } else {
       if (full) FSGC_TRI_LAUNCH(dev::kMaxTriJobs, true); else FSGC_TRI_LAUNCH(dev::kMaxTriJobs, false);
     }
