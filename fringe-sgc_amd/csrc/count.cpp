@@ -181,9 +181,9 @@ std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast)
       for (auto& gr : groups) {
         const auto& j0 = ps[gr.idx[0]].job;
         const int k0 = (j0.cls == kTriangle && j0.kind == kTriTailedClique) ? j0.p[3] : -1;
-        // launches of the full fc3 carry at most two motifs: their per-wedge formula
-        // dominates, and the 8-motif variant of that kernel spills to scratch
-        const int cap = needs_full(j0) ? 2 : kMaxPivotJobs;
+        // pivot launches of the full fc3 carry at most two motifs: their per-wedge
+        // formula dominates, and the 8-motif variant of that kernel spills to scratch
+        const int cap = (j0.cls == kWedge && needs_full(j0)) ? 2 : kMaxPivotJobs;
         if ((int)gr.idx.size() < cap && group_tier(j0) == tier && (key < 0 || k0 < 0 || key == k0)) {
           gr.idx.push_back(i);
           placed = true;

@@ -30,6 +30,7 @@ struct Query {
   fsgc::Prepared p;
   fsgc_query out{};
   bool done = false;
+  std::string err;  // message of a failed query
 };
 
 }  // namespace
@@ -49,6 +50,7 @@ struct fsgc_server {
   int inflight = 0;
   fsgc_server_stats st{};
   std::vector<std::thread> workers;
+  std::vector<hipStream_t> streams;  // one non-blocking stream per slot
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double clock_ms() const
@@ -125,18 +127,14 @@ bool fsgc_server::pick(std::vector<Query*>* batch, int* grid, size_t* scratch)
 void fsgc_server::worker(int slot)
 {
   (void)hipSetDevice(device);
-  hipStream_t stream = nullptr;
-  const bool have_stream = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess;
+  hipStream_t stream = streams[slot];
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
     std::vector<Query*> batch;
     int grid = 0;
     size_t scratch = 0;
-    while (!(have_stream && pick(&batch, &grid, &scratch))) {
-      if (stopping && (pending.empty() || !have_stream)) {
-        if (have_stream) (void)hipStreamDestroy(stream);
-        return;
-      }
+    while (!pick(&batch, &grid, &scratch)) {
+      if (stopping && pending.empty()) return;
       cv_work.wait(lk);
     }
     inflight_scratch += scratch;
@@ -163,6 +161,7 @@ void fsgc_server::worker(int slot)
     for (size_t i = 0; i < batch.size(); i++) {
       Query* q = batch[i];
       q->out.status = rc;
+      if (rc) q->err = err;
       if (!rc) {
         fsgc::finish(ps[i], res[i], g->dg.n, g->dg.m, &q->out.result);
         q->out.result.kernel_ms = ms;
@@ -206,6 +205,17 @@ int fsgc_server_create(const fsgc_server_config* cfg, fsgc_server** out)
   }
   s->st.slots = s->cfg.slots;
   s->st.scratch_cap = s->cfg.scratch_cap;
+  for (int i = 0; i < s->cfg.slots; i++) {
+    hipStream_t st = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
+      delete s;
+      fsgc::set_error(std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+      return -5;
+    }
+    s->streams.push_back(st);
+  }
   for (int i = 0; i < s->cfg.slots; i++) s->workers.emplace_back([s, i] { s->worker(i); });
   *out = s;
   return 0;
@@ -254,8 +264,8 @@ static int take(fsgc_server* s, std::map<uint64_t, Query*>::iterator it, fsgc_qu
   if (out) *out = q->out;
   s->live.erase(it);
   const int status = q->out.status;
+  if (status) fsgc::set_error(status == -12 ? "query needs more device scratch than the server's cap" : q->err);
   delete q;
-  if (status) fsgc::set_error(status == -12 ? "query needs more device scratch than the server's cap" : "count failed");
   return status;
 }
 
@@ -300,6 +310,7 @@ void fsgc_server_destroy(fsgc_server* s)
   }
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
+  for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
   for (auto& kv : s->live) delete kv.second;
   delete s;
 }
