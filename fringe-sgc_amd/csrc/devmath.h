@@ -309,10 +309,58 @@ __device__ __noinline__ u128 fc3_small(int fx, int fy, int fz, int x, int y, int
   return addx(addx(mulx(b0, (u128)t0, fl), mulx(b1, (u128)t1, fl), fl), mulx(b2, (u128)t2, fl), fl);
 }
 
+// C(n, k) for k <= 2 with the binom conventions (0 when n < k), as a u64
+__device__ __forceinline__ ull sbin(int n, int k)
+{
+  if (n < k) return 0;
+  if (k == 0) return 1;
+  if (k == 1) return (ull)n;
+  return ((ull)n * (ull)(n - 1)) >> 1;
+}
+
+// fc3_collapsed with f_Y = FY and f_Z = FZ (each at most 2) fixed at compile time:
+// the loops unroll and the small binomials become inline u64 expressions; only the
+// C(n_X - c, f_X) stay general (one per c = 0 .. FY + FZ).
+template <int FY, int FZ>
+__device__ __noinline__ u128 fc3_fixed(int fx, int x, int y, int z, int xy, int xz, int yz, int xyz, uint32_t& fl)
+{
+  const int nx = x + xy + xz + xyz;
+  u128 b[FY + FZ + 1];
+#pragma unroll
+  for (int c = 0; c <= FY + FZ; c++) b[c] = binom(nx - c, fx, fl);
+  u128 count = 0;
+#pragma unroll
+  for (int k1 = 0; k1 <= FY; k1++) {
+#pragma unroll
+    for (int k2 = 0; k2 <= FZ; k2++) {
+      u128 s1 = 0, s2 = 0;
+#pragma unroll
+      for (int i = 0; i <= k1; i++)
+        s1 = addx(s1, mulx((u128)sbin(xyz, i) * sbin(xy, k1 - i), (u128)sbin(xz + xyz - i, k2), fl), fl);
+#pragma unroll
+      for (int j = 0; j <= FY - k1; j++)
+        s2 = addx(s2, mulx((u128)sbin(yz, j) * sbin(y, FY - k1 - j), (u128)sbin(z + yz - j, FZ - k2), fl), fl);
+      count = addx(count, mulx(mulx(b[k1 + k2], s1, fl), s2, fl), fl);
+    }
+  }
+  return count;
+}
+
 __device__ __forceinline__ u128 fc3_xyz(int fx, int fy, int fz, int x, int y, int z, int xy, int xz, int yz, int xyz,
                                         uint32_t& fl)
 {
-  if (fy <= 1 && fz <= 1 && (x | y | z | xy | xz | yz | xyz) >= 0) return fc3_small(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+  if ((x | y | z | xy | xz | yz | xyz) >= 0) {
+    if (fy <= 1 && fz <= 1) return fc3_small(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+    if (fy <= 2 && fz <= 2) {
+      switch (fy * 3 + fz) {
+        case 2: return fc3_fixed<0, 2>(fx, x, y, z, xy, xz, yz, xyz, fl);
+        case 5: return fc3_fixed<1, 2>(fx, x, y, z, xy, xz, yz, xyz, fl);
+        case 6: return fc3_fixed<2, 0>(fx, x, y, z, xy, xz, yz, xyz, fl);
+        case 7: return fc3_fixed<2, 1>(fx, x, y, z, xy, xz, yz, xyz, fl);
+        default: return fc3_fixed<2, 2>(fx, x, y, z, xy, xz, yz, xyz, fl);
+      }
+    }
+  }
   return fc3_collapsed(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
 }
 

@@ -360,25 +360,12 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           __builtin_amdgcn_wave_barrier();
         }
-        // walk the window; the loads of the next 64 positions are issued before the
-        // current ones are evaluated
-        unsigned nd = 0;
-        int ntuv = 0, ndv = 0;
-        if (q_lo + lane < q_hi) {
-          nd = D[adj[ub + q_lo + lane]];
-          ntuv = tsup[ub + q_lo + lane];
-          ndv = adjdeg[ub + q_lo + lane];
-        }
+        // walk the window
         for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
           const int q = q0 + lane;
-          const unsigned draw = nd;
-          const int tuv = ntuv, dv = ndv;
-          if (q + kWave < q_hi) {
-            nd = D[adj[ub + q + kWave]];
-            ntuv = tsup[ub + q + kWave];
-            ndv = adjdeg[ub + q + kWave];
-          }
           if (q < q_hi) {
+            const unsigned draw = D[adj[ub + q]];
+            const int tuv = tsup[ub + q], dv = adjdeg[ub + q];
             const unsigned dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
             const bool tri = dv_ & 1u;
             const int cvw = (int)(dv_ >> 1);
