@@ -163,10 +163,12 @@ struct WedgeEval {
 template <bool kFull>
 __device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t& fl)
 {
-  if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
   if constexpr (kFull) {
+    if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
     if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
+  // single-vertex anchor sets only (the regions are non-negative for every wedge
+  // and triangle that passes the filters)
   return fc3_singletons(fr, venn, fl);
 }
 
@@ -267,21 +269,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       if (lane == 0 && x > w) { atomicMax(&D[x], e0); atomicOr(&D[x], 1u); }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
       if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 16 : 0);
-      // four list entries per lane in flight before their atomics
-      int k = s + lane;
-      for (; k + 3 * kWave < xe; k += 4 * kWave) {
-        const int y0 = adj[k], y1 = adj[k + kWave], y2 = adj[k + 2 * kWave], y3 = adj[k + 3 * kWave];
-        atomicMax(&D[y0], e0);
-        atomicMax(&D[y1], e0);
-        atomicMax(&D[y2], e0);
-        atomicMax(&D[y3], e0);
-        atomicAdd(&D[y0], 2u);
-        atomicAdd(&D[y1], 2u);
-        atomicAdd(&D[y2], 2u);
-        atomicAdd(&D[y3], 2u);
-        bytes += 80;
-      }
-      for (; k < xe; k += kWave) {
+      for (int k = s + lane; k < xe; k += kWave) {
         const int y = adj[k];
         atomicMax(&D[y], e0);
         atomicAdd(&D[y], 2u);

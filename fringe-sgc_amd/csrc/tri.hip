@@ -50,10 +50,12 @@ __device__ __forceinline__ int lbound(const int* __restrict__ a, int l, int r, i
 template <bool kFull>
 __device__ __forceinline__ u128 fc3_tri(const int* fr, const int* venn, uint32_t& fl)
 {
-  if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
   if constexpr (kFull) {
+    if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
     if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
+  // single-vertex anchor sets only (the regions are non-negative for every wedge
+  // and triangle that passes the filters)
   return fc3_singletons(fr, venn, fl);
 }
 
