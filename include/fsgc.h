@@ -71,7 +71,8 @@ typedef struct fsgc_tune {
   int32_t grid;        /* persistent workgroups; 0 = tuned or default (4-8 per CU) */
   int32_t chunk;       /* roots per work item; 0 = tuned or default */
   int32_t hub_degree;  /* roots above this degree get items of their own; 0 = tuned or default */
-  int32_t lds_cap;     /* longest neighbour list staged in LDS; 0 = default */
+  int32_t lds_cap;     /* reserved, ignored: each kernel shape fixes its LDS window (pivot: 4096 positions
+                          per wave for 256/512 threads, 2048 for 1024) */
   int32_t slice_mod;   /* > 1: count only roots u with u % slice_mod == slice_rem */
   int32_t slice_rem;
   int32_t schedule;    /* 0: fastest schedule per class; 1: per-root schedule (the reference's) for every class */
