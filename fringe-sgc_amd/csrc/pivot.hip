@@ -549,8 +549,14 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
     j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m;
     need_c3 = need_c3 || (j.kind == kWedgeGeneral);
   }
-  // work items: pivots split into centre ranges of bounded walk cost
-  const long long budget = std::max<long long>(1 << 16, (long long)t.chunk << 12);
+  // work items: pivots split into centre ranges of bounded walk cost. Every item of a
+  // split pivot repeats its 2-hop scatter (phase A), so a pivot is split only when its
+  // cost exceeds an eighth of one workgroup's share of the whole launch; heaviest-first
+  // order then balances the rest.
+  long long total_cost = 0;
+  for (int w = 0; w < g->n; w++) total_cost += g->h_piv_cost[w];
+  const long long budget = std::max<long long>(std::max<long long>(1 << 16, (long long)t.chunk << 12),
+                                               total_cost / ((long long)pivot_grid(t) * 8));
   std::vector<dev::PivotItem> items;
   std::vector<long long> icost;
   for (int w = 0; w < g->n; w++) {
