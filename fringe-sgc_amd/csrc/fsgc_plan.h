@@ -52,6 +52,7 @@ struct DeviceGraph {
   bool derived_ready = false;   // lists built too
   int32_t* tsup = nullptr;      // m: |N(u) & N(x)| per directed edge
   int32_t* adjdeg = nullptr;    // m: degree of adj[e] (read in CSR order instead of through idx)
+  int32_t* adjhub = nullptr;    // m: LDS slot of adj[e] in the pivot kernel when it is a hub, else ~adj[e]
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
   long long tl_total = 0;

@@ -34,11 +34,19 @@
 namespace fsgc {
 namespace dev {
 
-// launch shapes of the pivot kernel: waves per workgroup and LDS counters per wave
+// launch shapes of the pivot kernel: waves per workgroup and LDS c3 counters per wave
 // (positions of N(u)); 4 x 4096 (64 KB, two workgroups per CU), 8 x 4096 and
-// 16 x 2048 (128 KB, one workgroup per CU with 8 or 16 waves sharing one D array)
+// 16 x 2048 (128 KB, one workgroup per CU with 8 or 16 waves sharing one D array);
+// launches without c3 use a 16-wave shape whose LDS holds hub counters instead
 constexpr int kPivWavesMax = 16;
 constexpr int kMaxPivotJobs = 8;
+// Hubs: the kHubSlots highest-degree vertices, where most 2-hop paths end (about half
+// of them on R-MAT s20). Launches without c3 need no LDS windows, so their 16-wave
+// shape keeps the hub counters in LDS (160 KB) and only the other vertices' counters
+// in the global per-workgroup array D. (Launches with c3 keep their 128 KB of windows:
+// 8184 hubs in the remaining LDS measured 2% slower than none, the lanes of a wave
+// diverging between the two kinds of counter.)
+constexpr int kHubSlots = 40928;
 
 struct PivotItem {
   int32_t w;   // pivot
@@ -220,10 +228,11 @@ __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u,
   return true;
 }
 
-template <int NJ, bool kFull, int kPivWaves, int kECap>
+template <int NJ, bool kFull, int kPivWaves, int kECap, int kHub>
 __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
                                                            const int* __restrict__ tsup,
                                                            const int* __restrict__ adjdeg,
+                                                           const int* __restrict__ adjhub,
                                                            const long long* __restrict__ tl_off,
                                                            const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
@@ -231,6 +240,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
                                                            unsigned* __restrict__ dbuf, long long dstride)
 {
   __shared__ unsigned s_e[kPivWaves][kECap];
+  __shared__ unsigned s_hub[kHub > 0 ? kHub : 1];
   __shared__ int s_item, s_next;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -246,6 +256,10 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   // also leaves it zero for the next launch.
   unsigned epoch = 0;
   const long long dn = dstride;
+  // the hub counters are cleared before every item (the barrier after the item's
+  // fetch orders the clearing before its scatter)
+  if (kHub > 0)
+    for (int i = threadIdx.x; i < kHub; i += kPivWaves * kWave) s_hub[i] = 0u;
 
   for (;;) {
     if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; }
@@ -263,17 +277,33 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     const unsigned e0 = epoch << 24;
 
     // Phase A: D[y] += 2 for every path w - x - y with y > w; D[x] |= 1 for x in N(w).
+    // With hub slots, adjhub[e] names the LDS counter of a hub (>= 0) or ~y otherwise.
     for (int i = wb + wave; i < we; i += kPivWaves) {
       const int x = adj[i];
       const int xb = g.beg(x), xe = g.end(x);
-      if (lane == 0 && x > w) { atomicMax(&D[x], e0); atomicOr(&D[x], 1u); }
+      if (lane == 0 && x > w) {
+        const int hx = kHub > 0 ? adjhub[i] : ~x;
+        if (kHub > 0 && hx >= 0) {
+          atomicOr(&s_hub[hx], 1u);
+          bytes += 4;
+        } else {
+          atomicMax(&D[~hx], e0);
+          atomicOr(&D[~hx], 1u);
+          bytes += kHub > 0 ? 20 : 16;
+        }
+      }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1)) + (x > w ? 16 : 0);
+      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1));
       for (int k = s + lane; k < xe; k += kWave) {
-        const int y = adj[k];
-        atomicMax(&D[y], e0);
-        atomicAdd(&D[y], 2u);
-        bytes += 20;
+        const int h = kHub > 0 ? adjhub[k] : ~adj[k];
+        if (kHub > 0 && h >= 0) {
+          atomicAdd(&s_hub[h], 2u);
+          bytes += 4;
+        } else {
+          atomicMax(&D[~h], e0);
+          atomicAdd(&D[~h], 2u);
+          bytes += 20;
+        }
       }
     }
     __syncthreads();
@@ -298,8 +328,10 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       const bool c3on = need_c3 && tuw > 0;
       const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
       // the walk past w in windows of kECap positions, each with its c3 counters in LDS
-      for (int q_lo = rq + 1; q_lo < du; q_lo += kECap) {
-        const int q_hi = min(du, q_lo + kECap);
+      // (without c3 the whole walk is one window)
+      const int wstep = c3on ? kECap : du;
+      for (int q_lo = rq + 1; q_lo < du; q_lo += wstep) {
+        const int q_hi = min(du, q_lo + wstep);
         const bool last = q_hi == du;
         if (c3on) {
           for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
@@ -352,13 +384,20 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
           const int q = q0 + lane;
           if (q < q_hi) {
-            const unsigned draw = D[adj[ub + q]];
+            const int h = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
+            unsigned dv_;
+            if (kHub > 0 && h >= 0) {
+              dv_ = s_hub[h];
+            } else {
+              const unsigned draw = D[~h];
+              dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
+              bytes += 4;
+            }
             const int tuv = tsup[ub + q], dv = adjdeg[ub + q];
-            const unsigned dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
             const bool tri = dv_ & 1u;
             const int cvw = (int)(dv_ >> 1);
             const int c3 = c3on ? (int)E[q - q_lo] : 0;
-            bytes += 16;
+            bytes += 12;
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
               if (j < njobs) {
@@ -377,6 +416,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    if (kHub > 0)
+      for (int i = threadIdx.x; i < kHub; i += kPivWaves * kWave) s_hub[i] = 0u;
   }
   // leave D zero for the next launch
   if (epoch > 0) {
@@ -398,6 +439,13 @@ __global__ void k_adjdeg(const int* __restrict__ idx, const int* __restrict__ ad
     const int x = adj[e];
     out[e] = idx[x + 1] - idx[x];
   }
+}
+
+// hub slot (or ~vertex) of the neighbour at every CSR position
+__global__ void k_adjhub(const int* __restrict__ adj, long long m, const int* __restrict__ slot, int* __restrict__ out)
+{
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (long long)gridDim.x * blockDim.x)
+    out[e] = slot[adj[e]];
 }
 
 // cost of pivot w: sum of the degrees of its neighbours
@@ -425,11 +473,13 @@ void device_derived_free(DeviceGraph* g)
 {
   if (g->tsup) (void)hipFree(g->tsup);
   if (g->adjdeg) (void)hipFree(g->adjdeg);
+  if (g->adjhub) (void)hipFree(g->adjhub);
   if (g->tl_off) (void)hipFree(g->tl_off);
   if (g->tl) (void)hipFree(g->tl);
   if (g->piv_cost) (void)hipFree(g->piv_cost);
   g->tsup = nullptr;
   g->adjdeg = nullptr;
+  g->adjhub = nullptr;
   g->tl_off = nullptr;
   g->tl = nullptr;
   g->piv_cost = nullptr;
@@ -467,6 +517,24 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     HIP_TRY(hipMalloc((void**)&g->adjdeg, sizeof(int) * std::max<long long>(1, m)));
     hipLaunchKernelGGL(dev::k_adjdeg, dim3(cus * 8), dim3(256), 0, stream, g->idx, g->adj, m, g->adjdeg);
     HIP_TRY(hipGetLastError());
+    // hub slots: the kHubSlots vertices of highest degree (ties by id) get LDS slots
+    // 0, 1, ... in that order; every other vertex v is stored as ~v
+    {
+      std::vector<int> start((size_t)g->maxdeg + 2, 0), slot((size_t)std::max(1, n));
+      for (int v = 0; v < n; v++) start[(size_t)(g->maxdeg - (g->h_idx[v + 1] - g->h_idx[v])) + 1]++;
+      for (size_t i = 1; i < start.size(); i++) start[i] += start[i - 1];
+      for (int v = 0; v < n; v++) {
+        const int r = start[(size_t)(g->maxdeg - (g->h_idx[v + 1] - g->h_idx[v]))]++;
+        slot[v] = r < dev::kHubSlots ? r : ~v;
+      }
+      int* d_slot = nullptr;
+      HIP_TRY(sc.get(&d_slot, sizeof(int) * (size_t)std::max(1, n)));
+      HIP_TRY(hipMalloc((void**)&g->adjhub, sizeof(int) * std::max<long long>(1, m)));
+      HIP_TRY(hipMemcpyAsync(d_slot, slot.data(), sizeof(int) * (size_t)std::max(1, n), hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(dev::k_adjhub, dim3(cus * 8), dim3(256), 0, stream, g->adj, m, d_slot, g->adjhub);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(stream));  // the host slot array goes out of scope
+    }
     hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
                        (const long long*)nullptr, (int*)nullptr, d_head);
     HIP_TRY(hipGetLastError());
@@ -475,7 +543,7 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     g->h_piv_cost.resize(n);
     HIP_TRY(hipMemcpyAsync(g->h_piv_cost.data(), g->piv_cost, sizeof(long long) * n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
-    g->derived_bytes = 2 * sizeof(int) * (size_t)m + sizeof(long long) * (size_t)n;
+    g->derived_bytes = 3 * sizeof(int) * (size_t)m + sizeof(long long) * (size_t)n;
     g->support_ready = true;
   }
   if (lists) {
@@ -609,19 +677,21 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const int* adj = g->adj;
   const int* tsup = g->tsup;
   const int* adjdeg = g->adjdeg;
+  const int* adjhub = g->adjhub;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   lp->nitems = nitems;
   lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
-#define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP)                                                                   \
-  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
-                     tsup, adjdeg, tl_off, tl, d_jobs, nj, c3, d_items + first, count, head, dbuf, dstride)
+#define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB)                                                                   \
+  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP, HUB>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
+                     tsup, adjdeg, adjhub, tl_off, tl, d_jobs, nj, c3, d_items + first, count, head, dbuf, dstride)
 #define FSGC_PIVOT_SHAPE(NJ, FULL)                                          \
   do {                                                                      \
-    if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048);                 \
-    else if (waves == 8) FSGC_PIVOT_LAUNCH(NJ, FULL, 8, 4096);              \
-    else FSGC_PIVOT_LAUNCH(NJ, FULL, 4, 4096);                              \
+    if (waves == 16 && c3) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048, 0);         \
+    else if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 1, dev::kHubSlots); \
+    else if (waves == 8) FSGC_PIVOT_LAUNCH(NJ, FULL, 8, 4096, 0);           \
+    else FSGC_PIVOT_LAUNCH(NJ, FULL, 4, 4096, 0);                           \
   } while (0)
     if (nj == 1) {
       if (full) FSGC_PIVOT_SHAPE(1, true); else FSGC_PIVOT_SHAPE(1, false);

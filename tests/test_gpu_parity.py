@@ -151,3 +151,31 @@ def test_triangle_rank_schedule_matches_per_root(gpu):
             assert (a.sg, a.core_matches, a.exact) == (b.sg, b.core_matches, b.exact), (gpu.live_fields(mo), spec)
             o = oracle.count(mo, ni, nl)
             same(a, o, f"rank {gpu.live_fields(mo)} on {spec}")
+
+
+def test_wedge_pivot_hub_and_global_counters(gpu):
+    """Graphs with more vertices than the pivot kernel has hub slots (40928). Launches
+    without c3 (the closed-form wedge kinds) then keep the 2-hop counters of the
+    highest-degree vertices in LDS and the others in global memory; launches with c3,
+    and the 4- and 8-wave shapes, keep all of them in global memory. Each shape, and a
+    batch that mixes c3 and closed-form motifs, matches the oracle and the per-root
+    schedule."""
+    for spec in ("er:50000:200000:41", "chunglu:50000:2.1:8:500:43"):
+        ni, nl = gpu.generate_csr(spec)
+        g = gpu.Graph.from_csr(ni, nl)
+        mos = [gpu.preprocess(*C2_MOTIF)[0], gpu.preprocess_pattern(13)[0], gpu.preprocess_pattern(15)[0]]
+        refs = []
+        for mo in mos:
+            print("  ", spec, gpu.live_fields(mo), flush=True)
+            o = oracle.count(mo, ni, nl)
+            a = g.count(mo)
+            same(a, o, f"pivot {gpu.live_fields(mo)} on {spec}")
+            b = g.count(mo, gpu.Tune(schedule=1))
+            assert (b.sg, b.core_matches) == (a.sg, a.core_matches)
+            for block in (256, 512):
+                c = g.count(mo, gpu.Tune(block=block))
+                assert (c.exact, c.sg, c.core_matches) == (a.exact, a.sg, a.core_matches), block
+            refs.append(a)
+        batch, _ = g.count_batch(mos)
+        for a, b in zip(refs, batch):
+            assert (a.exact, a.sg, a.core_matches, a.flags) == (b.exact, b.sg, b.core_matches, b.flags)
