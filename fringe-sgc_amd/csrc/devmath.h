@@ -238,13 +238,25 @@ static __device__ void flush(LaneAcc& a, Accum* out)
 
 // ------------------------------------------------------------------ fringe formulas
 
-// binomial with the reference's conventions, inline for k <= 2
+// binomial with the reference's conventions (the same values as ncr), inline for
+// k <= 2, and for k = 3 (n < 2^21) and k = 4 (n < 2^16), where the falling factorial
+// fits u64 and the reference's u64 loop does not wrap: the division by k! is exact,
+// a shift followed by the inverse of 3 modulo 2^64 (3 * 0xAA..AB = 1 mod 2^64)
 __device__ __forceinline__ u128 binom(int n, int k, uint32_t& fl)
 {
+  constexpr ull kInv3 = 0xAAAAAAAAAAAAAAABull;
   if (k == 0) return n >= 0 ? 1 : 0;
   if (n < k) return 0;
   if (k == 1) return (ull)n;
   if (k == 2) return ((ull)n * (ull)(n - 1)) >> 1;
+  if (k == 3 && n < (1 << 21)) {
+    const ull N = (ull)n;
+    return ((N * (N - 1) * (N - 2)) >> 1) * kInv3;
+  }
+  if (k == 4 && n < (1 << 16)) {
+    const ull N = (ull)n;
+    return ((N * (N - 1) * (N - 2) * (N - 3)) >> 3) * kInv3;
+  }
   return ncr(n, k, fl);
 }
 
@@ -487,11 +499,11 @@ static __device__ __noinline__ u128 pair_formula(int da, int db, int c_ab, int c
   u128 acc = 0;
   for (int j = 0; j <= bound; j++) {
     const int k = f_a - j;
-    u128 term = mulx(ncr(ca, k, fl), ncr(a_ab + cb - j, f_b, fl), fl);
-    if (f_a != f_b) term = addx(term, mulx(ncr(cb, k, fl), ncr(a_ab + ca - j, f_b, fl), fl), fl);
-    acc = addx(acc, mulx(term, ncr(a_ab, j, fl), fl), fl);
+    u128 term = mulx(binom(ca, k, fl), binom(a_ab + cb - j, f_b, fl), fl);
+    if (f_a != f_b) term = addx(term, mulx(binom(cb, k, fl), binom(a_ab + ca - j, f_b, fl), fl), fl);
+    acc = addx(acc, mulx(term, binom(a_ab, j, fl), fl), fl);
   }
-  return mulx(acc, ncr(c_ab, f_ab, fl), fl);
+  return mulx(acc, binom(c_ab, f_ab, fl), fl);
 }
 
 }  // namespace dev
