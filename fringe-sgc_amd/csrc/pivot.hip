@@ -374,7 +374,16 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
               const int who = __builtin_ctzll(bm);
               bm &= bm - 1;
               const long long B = bcast(lb, who), Ee = bcast(le, who);
-              for (long long k = B + lane; k < Ee; k += kWave) atomicAdd(&E[tl[k] - q_lo], 1u);
+              // eight independent loads in flight per lane before their increments
+              long long k = B + lane;
+              for (; k + 7 * kWave < Ee; k += 8 * kWave) {
+                int a[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) a[r] = tl[k + r * kWave];
+#pragma unroll
+                for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
+              }
+              for (; k < Ee; k += kWave) atomicAdd(&E[tl[k] - q_lo], 1u);
             }
           }
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
