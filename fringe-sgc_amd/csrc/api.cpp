@@ -129,6 +129,33 @@ int fsgc_graph_info(const fsgc_graph* g, int32_t* nodes, int32_t* edges, int32_t
   return 0;
 }
 
+int fsgc_graph_relabel(const fsgc_graph* g, int32_t order, fsgc_graph** out, int32_t* map, double* relabel_ms)
+{
+  if (!g || !out) return fail(-22, "null argument");
+  if (order != FSGC_RELABEL_DEGREE_DESC && order != FSGC_RELABEL_DEGREE_ASC) return fail(-22, "unknown relabel order");
+  int rc = ensure_init();
+  if (rc) return rc;
+  fsgc_graph* r = new fsgc_graph();
+  std::vector<int32_t> h_map;
+  std::string err;
+  double ms = 0;
+  rc = fsgc::device_graph_relabel(g->dg, order, &r->dg, map ? &h_map : nullptr, &ms, &err);
+  if (rc) { fsgc::device_graph_free(&r->dg); delete r; return fail(rc, err); }
+  if (map) std::memcpy(map, h_map.data(), sizeof(int32_t) * h_map.size());
+  if (relabel_ms) *relabel_ms = ms;
+  *out = r;
+  return 0;
+}
+
+int fsgc_graph_copy_csr(const fsgc_graph* g, int32_t* nindex, int32_t* nlist)
+{
+  if (!g || !nindex || (g->dg.m > 0 && !nlist)) return fail(-22, "null argument");
+  std::memcpy(nindex, g->dg.h_idx.data(), sizeof(int32_t) * ((size_t)g->dg.n + 1));
+  if (g->dg.m && hipMemcpy(nlist, g->dg.adj, sizeof(int32_t) * (size_t)g->dg.m, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(-5, "copy of the neighbour list failed");
+  return 0;
+}
+
 void fsgc_graph_free(fsgc_graph* g)
 {
   if (!g) return;

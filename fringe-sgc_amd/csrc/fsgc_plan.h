@@ -138,6 +138,11 @@ int device_init(int device, std::string* err);
 int device_graph_upload(const HostGraph& g, DeviceGraph* dg, std::string* err);
 int device_graph_upload_async(const HostGraph& g, DeviceGraph* dg, DevScratch* sc, std::string* err);
 void device_graph_free(DeviceGraph* dg);
+// relabel.hip: a new resident graph renumbered by degree (order 0: highest first, ties
+// by the larger id first, as the reference's relabelByDegree; 1: lowest first, ties by
+// the smaller id first); h_map (optional) receives old -> new ids, ms the device time
+int device_graph_relabel(const DeviceGraph& src, int order, DeviceGraph* dst, std::vector<int32_t>* h_map,
+                         double* ms, std::string* err);
 // Runs the given jobs (each bound to a graph) in one persistent launch on `stream`
 // and returns one result per job. kernel_ms receives the device time of the launch.
 size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t);

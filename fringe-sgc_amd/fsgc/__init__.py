@@ -64,6 +64,7 @@ class ServerStats(C.Structure):
 
 
 SCHEDULE_NAMES = ["per-root", "pivot", "rank", "edge-pass"]
+RELABEL_ORDERS = {"desc": 0, "asc": 1}
 
 
 def _load():
@@ -78,6 +79,8 @@ def _load():
         "fsgc_graph_validate": (C.c_int, [C.c_void_p]),
         "fsgc_graph_info": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
         "fsgc_graph_free": (None, [C.c_void_p]),
+        "fsgc_graph_relabel": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_void_p), P(C.c_int32), P(C.c_double)]),
+        "fsgc_graph_copy_csr": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "fsgc_count": (C.c_int, [C.c_void_p, C.c_char_p, P(Tune), P(_Result)]),
         "fsgc_count_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P(Tune), P(_Result), P(C.c_double)]),
         "fsgc_preprocess": (C.c_int, [C.c_int, C.c_int, P(C.c_int), C.c_char_p, C.c_char_p, C.c_size_t,
@@ -206,6 +209,30 @@ class Graph:
 
     def validate(self):
         _check(lib.fsgc_graph_validate(self._h))
+
+    def relabel(self, order: str = "desc"):
+        """A new resident graph renumbered by degree (relabelByDegree, fringeCount.cu:
+        1221-1288): "desc" highest degree first, ties by the larger id (the reference's
+        order); "asc" lowest first, ties by the smaller id. Returns (graph, map, ms) with
+        map[old] = new as an int32 array."""
+        import numpy as np
+        n, _, _ = self.info()
+        mp = np.empty(n, dtype=np.int32)
+        h = C.c_void_p()
+        ms = C.c_double()
+        _check(lib.fsgc_graph_relabel(self._h, RELABEL_ORDERS[order], C.byref(h),
+                                      mp.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(ms)))
+        return Graph(h.value), mp, ms.value
+
+    def csr(self):
+        """The resident CSR copied to the host: (nindex, nlist) int32 arrays."""
+        import numpy as np
+        n, m, _ = self.info()
+        ni = np.empty(n + 1, dtype=np.int32)
+        nl = np.empty(max(1, m), dtype=np.int32)
+        _check(lib.fsgc_graph_copy_csr(self._h, ni.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       nl.ctypes.data_as(C.POINTER(C.c_int32))))
+        return ni, nl[:m]
 
     def count(self, mo: bytes, tune: Tune | None = None) -> Result:
         assert len(mo) == MO_BYTES

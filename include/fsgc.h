@@ -11,6 +11,7 @@
  *   fsgc_graph_load_egr  readECLgraph + host-to-device copy   src/ECLgraph.h:58-85, src/fringeCount.cu:2285-2297
  *   fsgc_graph_from_csr  the same from caller memory          src/fringeCount.cu:2292-2297
  *   fsgc_graph_free      freeECLgraph + cudaFree              src/ECLgraph.h:88-96, src/fringeCount.cu:2325-2329
+ *   fsgc_graph_relabel   relabelByDegree (into a new graph)    src/fringeCount.cu:1221-1288, 1065-1164
  *   fsgc_count           occurrences(mo, d_g, wl, ...)         src/fringeCount.cu:2199-2262
  *   fsgc_count_batch     (no counterpart: the reference counts one motif per process)
  *   fsgc_batch_*         (no counterpart: the same, captured once as a HIP graph and replayed)
@@ -88,6 +89,24 @@ int fsgc_graph_from_csr(int32_t nodes, int32_t edges, const int32_t* nindex, con
 int fsgc_graph_validate(const fsgc_graph* g);
 int fsgc_graph_info(const fsgc_graph* g, int32_t* nodes, int32_t* edges, int32_t* maxdeg);
 void fsgc_graph_free(fsgc_graph* g);
+
+/* Degree relabelling (relabelByDegree, fringeCount.cu:1221-1288, with its kernels
+   init_sort/create_map/create_edge_list/create_nindex/create_nlist, 1065-1164). The
+   reference renumbers the caller's device graph in place before a triangle-core count
+   (fringeCount.cu:1348), and skips that when the maximum degree is at most 16. Here the
+   caller's graph is left unchanged and *out receives a new resident graph in which
+   vertex u of g is vertex map[u]:
+     FSGC_RELABEL_DEGREE_DESC  degree highest first, ties by the larger id first (the
+                               reference's descending sort of (degree << idbits) + id)
+     FSGC_RELABEL_DEGREE_ASC   degree lowest first, ties by the smaller id first
+   Lists stay sorted. Counts on *out equal counts on g. map (n entries) and relabel_ms
+   (device time) may be NULL. */
+#define FSGC_RELABEL_DEGREE_DESC 0
+#define FSGC_RELABEL_DEGREE_ASC 1
+int fsgc_graph_relabel(const fsgc_graph* g, int32_t order, fsgc_graph** out, int32_t* map, double* relabel_ms);
+/* Copies a resident graph's CSR to the host (nindex: nodes + 1 entries, nlist: edges).
+   No reference counterpart: the reference never reads its device graph back. */
+int fsgc_graph_copy_csr(const fsgc_graph* g, int32_t* nindex, int32_t* nlist);
 
 /* Counts one motif (one persistent launch on the data graph, plus one on the motif
    graph for the automorphism sum, as the reference does). tune may be NULL. */

@@ -118,3 +118,26 @@ def test_config_graphs_match_committed_sha256():
         h.update(nl.astype("<i4").tobytes())
         assert h.hexdigest() == ref[spec]["sha256"], spec
         assert len(nl) == ref[spec]["directed_edges"]
+
+
+def test_relabel_order_is_the_reference_key_order():
+    """The host renumbering the relabel GPU tests compare against sorts vertices as the
+    reference's relabelByDegree does: keys (degree << idbits) + id sorted descending
+    (fringeCount.cu:1064-1075, 1241-1251), or ascending for the opposite order; and it
+    keeps every count (oracle on both numberings)."""
+    import oracle
+    from test_gpu_relabel import host_relabel
+    ni, nl = fsgc.generate_csr("rmat:8:8:0.57:0.19:0.19:3")
+    n = len(ni) - 1
+    deg = np.diff(ni).astype(np.uint64)
+    idbits = max(1, int(n - 1).bit_length())
+    keys = (deg << np.uint64(idbits)) | np.arange(n, dtype=np.uint64)
+    for order in ("desc", "asc"):
+        mp, a, b = host_relabel(ni, nl, order)
+        srt = np.sort(keys)[::-1] if order == "desc" else np.sort(keys)
+        perm = (srt & np.uint64((1 << idbits) - 1)).astype(np.int64)
+        assert np.array_equal(mp[perm], np.arange(n))
+        for p in (2, 7, 9, 23):
+            mo = fsgc.preprocess_pattern(p)[0]
+            x, y = oracle.count(mo, ni, nl), oracle.count(mo, a, b)
+            assert (x["exact"], x["sg"], x["aut"]) == (y["exact"], y["sg"], y["aut"]), (order, p)
