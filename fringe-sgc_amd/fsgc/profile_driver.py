@@ -4,7 +4,8 @@
 
 First a device-to-device copy of a known size (torch, `copy_`, the calibration
 kernel for FETCH_SIZE/WRITE_SIZE), then one untimed count (index build and
-automorphism sum) and one measured count of the workload. Prints what it ran.
+automorphism sum) and one measured count of the workload, on the graph renumbered
+by degree when the workload asks for it (as bench.py does). Prints what it ran.
 """
 import argparse
 import os
@@ -19,6 +20,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--relabel", default="auto", choices=["auto", "none", "desc", "asc"],
+                    help="degree relabelling before the counts (auto: the workload's choice in bench.py)")
     args = ap.parse_args()
     import torch
     import fsgc
@@ -33,6 +36,12 @@ def main():
     W = WORKLOADS[args.workload]
     fsgc.init(0)
     g = fsgc.Graph.generate(W["spec"])
+    order = W.get("relabel", "none") if args.relabel == "auto" else args.relabel
+    if order != "none":
+        g0 = g
+        g, _, ms = g0.relabel(order)
+        g0.free()
+        print(f"relabelled by degree ({order}) in {ms:.1f} ms", flush=True)
     mo, _ = fsgc.preprocess(*W["motif"])
     r0 = g.count(mo)
     print(f"warm count: index {r0.index_ms:.1f} ms kernel {r0.kernel_ms:.1f} ms", flush=True)
