@@ -55,6 +55,7 @@ struct DeviceGraph {
   int32_t* adjhub = nullptr;    // m: LDS slot of adj[e] in the pivot kernel when it is a hub, else ~adj[e]
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
+  uint16_t* tl16 = nullptr;     // the same lists in 16 bits (built when maxdeg <= 65536; pivot kernel)
   long long tl_total = 0;
   long long* piv_cost = nullptr;         // n: sum of neighbour degrees
   std::vector<long long> h_piv_cost;

@@ -228,13 +228,16 @@ __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u,
   return true;
 }
 
-template <int NJ, bool kFull, int kPivWaves, int kECap, int kHub>
+// TL: element type of the common-neighbour lists the kernel reads (positions in N(u):
+// uint16_t when every degree is at most 65536, which halves the list traffic of the c3
+// scatter, else int)
+template <int NJ, bool kFull, int kPivWaves, int kECap, int kHub, typename TL>
 __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __restrict__ idx, const int* __restrict__ adj,
                                                            const int* __restrict__ tsup,
                                                            const int* __restrict__ adjdeg,
                                                            const int* __restrict__ adjhub,
                                                            const long long* __restrict__ tl_off,
-                                                           const int* __restrict__ tl, const JobDesc* __restrict__ jobs,
+                                                           const TL* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
                                                            int nitems, int* __restrict__ head,
                                                            unsigned* __restrict__ dbuf, long long dstride)
@@ -341,33 +344,34 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           for (long long t0 = tb; t0 < te; t0 += kWave) {
             const long long t = t0 + lane;
             int xq = -1;
-            if (t < te) xq = tl[t];
+            if (t < te) xq = (int)tl[t];
             // each lane walks its own list; long lists are shared by the wave
             long long lb = 0, le = 0;
             if (xq >= 0) {
               const long long b0 = tl_off[ub + xq], e0 = tl_off[ub + xq + 1];
               long long lo = b0, hi = e0;  // first entry >= q_lo
-              while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
+              while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
               lb = lo;
               le = e0;
               if (!last) {  // first entry >= q_hi
                 hi = e0;
-                while (lo < hi) { const long long mid = (lo + hi) >> 1; if (tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
+                while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
                 le = lo;
               }
-              bytes += 20 + 8 * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) + (le - lb) * 4;
+              bytes += 16 + sizeof(TL) + (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) +
+                       (le - lb) * sizeof(TL);
             }
             const bool big = (le - lb) > 64;
             if (!big) {
               long long k = lb;
               for (; k + 4 <= le; k += 4) {
-                const int a0 = tl[k], a1 = tl[k + 1], a2 = tl[k + 2], a3 = tl[k + 3];
+                const int a0 = (int)tl[k], a1 = (int)tl[k + 1], a2 = (int)tl[k + 2], a3 = (int)tl[k + 3];
                 atomicAdd(&E[a0 - q_lo], 1u);
                 atomicAdd(&E[a1 - q_lo], 1u);
                 atomicAdd(&E[a2 - q_lo], 1u);
                 atomicAdd(&E[a3 - q_lo], 1u);
               }
-              for (; k < le; k++) atomicAdd(&E[tl[k] - q_lo], 1u);
+              for (; k < le; k++) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
             }
             ull bm = __ballot(big);
             while (bm) {
@@ -379,11 +383,11 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
               for (; k + 7 * kWave < Ee; k += 8 * kWave) {
                 int a[8];
 #pragma unroll
-                for (int r = 0; r < 8; r++) a[r] = tl[k + r * kWave];
+                for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
 #pragma unroll
                 for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
               }
-              for (; k < Ee; k += kWave) atomicAdd(&E[tl[k] - q_lo], 1u);
+              for (; k < Ee; k += kWave) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
             }
           }
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -441,6 +445,13 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   if (lane == 0) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
+// 16-bit copy of the common-neighbour lists (every entry below 65536)
+__global__ void k_narrow16(const int* __restrict__ tl, long long total, uint16_t* __restrict__ out)
+{
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x)
+    out[e] = (uint16_t)tl[e];
+}
+
 // degree of the neighbour at every CSR position
 __global__ void k_adjdeg(const int* __restrict__ idx, const int* __restrict__ adj, long long m, int* __restrict__ out)
 {
@@ -485,12 +496,14 @@ void device_derived_free(DeviceGraph* g)
   if (g->adjhub) (void)hipFree(g->adjhub);
   if (g->tl_off) (void)hipFree(g->tl_off);
   if (g->tl) (void)hipFree(g->tl);
+  if (g->tl16) (void)hipFree(g->tl16);
   if (g->piv_cost) (void)hipFree(g->piv_cost);
   g->tsup = nullptr;
   g->adjdeg = nullptr;
   g->adjhub = nullptr;
   g->tl_off = nullptr;
   g->tl = nullptr;
+  g->tl16 = nullptr;
   g->piv_cost = nullptr;
   g->derived_ready = false;
   g->support_ready = false;
@@ -575,6 +588,14 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
                        (const long long*)g->tl_off, g->tl, d_head + 1);
     HIP_TRY(hipGetLastError());
     g->derived_bytes += sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total;
+    // 16-bit copy of the lists for the pivot kernel: a position in N(u) is below
+    // d_u <= maxdeg, so it fits when maxdeg <= 65536
+    if (g->maxdeg <= 65536 && total > 0) {
+      HIP_TRY(hipMalloc((void**)&g->tl16, sizeof(uint16_t) * (size_t)total));
+      hipLaunchKernelGGL(dev::k_narrow16, dim3(cus * 8), dim3(256), 0, stream, (const int*)g->tl, total, g->tl16);
+      HIP_TRY(hipGetLastError());
+      g->derived_bytes += sizeof(uint16_t) * (size_t)total;
+    }
     g->derived_ready = true;
   }
   HIP_TRY(hipEventRecord(ev.b, stream));
@@ -689,18 +710,25 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const int* adjhub = g->adjhub;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
+  const uint16_t* tl16 = g->tl16;
   lp->nitems = nitems;
   lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
-#define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB)                                                                   \
-  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP, HUB>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, adj,  \
-                     tsup, adjdeg, adjhub, tl_off, tl, d_jobs, nj, c3, d_items + first, count, head, dbuf, dstride)
-#define FSGC_PIVOT_SHAPE(NJ, FULL)                                          \
-  do {                                                                      \
-    if (waves == 16 && c3) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 2048, 0);         \
-    else if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 1, dev::kHubSlots); \
-    else if (waves == 8) FSGC_PIVOT_LAUNCH(NJ, FULL, 8, 4096, 0);           \
-    else FSGC_PIVOT_LAUNCH(NJ, FULL, 4, 4096, 0);                           \
+#define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB, TL, TLP)                                                          \
+  hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP, HUB, TL>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, \
+                     adj, tsup, adjdeg, adjhub, tl_off, TLP, d_jobs, nj, c3, d_items + first, count, head, dbuf,   \
+                     dstride)
+#define FSGC_PIVOT_C3(NJ, FULL, W, CAP)                                   \
+  do {                                                                    \
+    if (tl16) FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, uint16_t, tl16);     \
+    else FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, int, tl);                 \
+  } while (0)
+#define FSGC_PIVOT_SHAPE(NJ, FULL)                                                     \
+  do {                                                                                 \
+    if (waves == 16 && c3) FSGC_PIVOT_C3(NJ, FULL, 16, 2048);                           \
+    else if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 1, dev::kHubSlots, int, tl); \
+    else if (waves == 8) FSGC_PIVOT_C3(NJ, FULL, 8, 4096);                              \
+    else FSGC_PIVOT_C3(NJ, FULL, 4, 4096);                                              \
   } while (0)
     if (nj == 1) {
       if (full) FSGC_PIVOT_SHAPE(1, true); else FSGC_PIVOT_SHAPE(1, false);
@@ -710,6 +738,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
       if (full) FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, true); else FSGC_PIVOT_SHAPE(dev::kMaxPivotJobs, false);
     }
 #undef FSGC_PIVOT_SHAPE
+#undef FSGC_PIVOT_C3
 #undef FSGC_PIVOT_LAUNCH
   };
   return 0;
