@@ -378,14 +378,45 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
               const int who = __builtin_ctzll(bm);
               bm &= bm - 1;
               const long long B = bcast(lb, who), Ee = bcast(le, who);
-              // eight independent loads in flight per lane before their increments
               long long k = B + lane;
-              for (; k + 7 * kWave < Ee; k += 8 * kWave) {
-                int a[8];
+              if constexpr (sizeof(TL) == 2) {
+                // 16-bit lists: four positions per 8-byte load, four loads in flight per
+                // lane. The head up to the first multiple of 4 (8-byte aligned, as tl
+                // itself is) and the tail past the last full group go one per lane.
+                const long long a4 = min(Ee, (B + 3) & ~3ll);
+                if (B + lane < a4) atomicAdd(&E[(int)tl[B + lane] - q_lo], 1u);
+                const long long n4 = (Ee - a4) >> 2;
+                const ushort4* __restrict__ v4 = reinterpret_cast<const ushort4*>(tl + a4);
+                long long g4 = lane;
+                for (; g4 + 3 * kWave < n4; g4 += 4 * kWave) {
+                  ushort4 r[4];
 #pragma unroll
-                for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
+                  for (int i = 0; i < 4; i++) r[i] = v4[g4 + i * kWave];
 #pragma unroll
-                for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
+                  for (int i = 0; i < 4; i++) {
+                    atomicAdd(&E[(int)r[i].x - q_lo], 1u);
+                    atomicAdd(&E[(int)r[i].y - q_lo], 1u);
+                    atomicAdd(&E[(int)r[i].z - q_lo], 1u);
+                    atomicAdd(&E[(int)r[i].w - q_lo], 1u);
+                  }
+                }
+                for (; g4 < n4; g4 += kWave) {
+                  const ushort4 r = v4[g4];
+                  atomicAdd(&E[(int)r.x - q_lo], 1u);
+                  atomicAdd(&E[(int)r.y - q_lo], 1u);
+                  atomicAdd(&E[(int)r.z - q_lo], 1u);
+                  atomicAdd(&E[(int)r.w - q_lo], 1u);
+                }
+                k = a4 + (n4 << 2) + lane;  // the tail, below
+              } else {
+                // eight independent loads in flight per lane before their increments
+                for (; k + 7 * kWave < Ee; k += 8 * kWave) {
+                  int a[8];
+#pragma unroll
+                  for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
+#pragma unroll
+                  for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
+                }
               }
               for (; k < Ee; k += kWave) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
             }
