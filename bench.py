@@ -5,8 +5,10 @@ Workload (default): R-MAT scale 20, edge factor 16, (a, b, c) = (0.57, 0.19, 0.1
 seed 7, symmetrised and deduplicated (1,048,576 vertices, 31,400,418 directed
 entries), resident in HBM; motif: wedge core with 3 fringe vertices on the centre and
 one on each endpoint (`8 0 1 1 2 1 3 1 4 1 5 0 6 2 7`). One step = one exact count
-of that motif over the whole graph (fsgc_count; the per-graph edge index is built
-once, before the timed region, and reported as index_build_ms).
+of that motif over the whole graph (fsgc_count). The graph is renumbered by ascending
+degree once at load (fsgc_graph_relabel, reported as relabel_ms; --relabel none
+counts on the generator's ids, with the same result), and the per-graph edge index
+is built once, before the timed region, and reported as index_build_ms.
 
 Metric: core matches per second (core embeddings whose fringe formula is evaluated).
 Also reported: the roofline of the dominant kernel (k_wedge_pivot for the default
@@ -17,7 +19,7 @@ root vertices, checked bit-exact against the GPU on the same slice. Beside it, t
 two other parts of the metric on small graphs: batched vs one-motif-per-launch motifs/s
 over the random-motif generator (seed 3) and the served query stream's p50/p99.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-extras]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-extras] [--relabel asc|desc|none]
     python bench.py --batch <graph spec> [--classes vertex,edge,...]   (plain vs batched)
     python bench.py --serve <spec>,<spec> [--rate Q] [--queries N]    (served stream)
 """
