@@ -358,11 +358,56 @@ __device__ __noinline__ u128 fc3_fixed(int fx, int x, int y, int z, int xy, int 
   return count;
 }
 
+// C(n, k) as a u64 with the binom conventions, for the cases binom evaluates inline:
+// k <= 2, k = 3 with n < 2^21, k = 4 with n < 2^16 (the caller checks the range)
+__device__ __forceinline__ ull ubin_small(int n, int k)
+{
+  constexpr ull kInv3 = 0xAAAAAAAAAAAAAAABull;
+  if (k == 0) return n >= 0 ? 1 : 0;
+  if (n < k) return 0;
+  const ull N = (ull)n;
+  if (k == 1) return N;
+  if (k == 2) return (N * (N - 1)) >> 1;
+  if (k == 3) return ((N * (N - 1) * (N - 2)) >> 1) * kInv3;
+  return ((N * (N - 1) * (N - 2) * (N - 3)) >> 3) * kInv3;
+}
+
+// fc3_small inline, when its three binomials C(n_X - c, f_X) are inline u64 values
+// (f_X <= 2, or f_X = 3 with n_X < 2^21, or f_X = 4 with n_X < 2^16). Then every
+// binomial is below 2^61 and every bracket below 2^63, so the three products and
+// their sum stay below 2^126: no flag can arise, and the value is fc3_small's. This
+// keeps the common wedge and triangle cases free of a call, whose register saves and
+// restores would go through scratch memory once per core match.
+__device__ __forceinline__ bool fc3_small_inline_ok(int fx, int nx)
+{
+  return fx <= 2 || (fx == 3 && nx < (1 << 21)) || (fx == 4 && nx < (1 << 16));
+}
+
+__device__ __forceinline__ u128 fc3_small_inline(int fx, int fy, int fz, int nx, int y, int z, int xy, int xz, int yz,
+                                                 int xyz)
+{
+  const ull b0 = ubin_small(nx, fx);
+  if ((fy | fz) == 0) return b0;
+  const ull b1 = ubin_small(nx - 1, fx);
+  if (fz == 0) return (u128)b0 * (ull)(y + yz) + (u128)b1 * (ull)(xy + xyz);
+  if (fy == 0) return (u128)b0 * (ull)(z + yz) + (u128)b1 * (ull)(xz + xyz);
+  const ull b2 = ubin_small(nx - 2, fx);
+  const long long zyz = z + yz, xzxyz = xz + xyz;
+  const ull t0 = (ull)((long long)y * zyz + (long long)yz * (zyz - 1));
+  const ull t1 = (ull)((long long)(xy + xyz) * zyz + xzxyz * (long long)(y + yz));
+  const ull t2 = (ull)((long long)xy * xzxyz + (long long)xyz * (xzxyz - 1));
+  return (u128)b0 * t0 + (u128)b1 * t1 + (u128)b2 * t2;
+}
+
 __device__ __forceinline__ u128 fc3_xyz(int fx, int fy, int fz, int x, int y, int z, int xy, int xz, int yz, int xyz,
                                         uint32_t& fl)
 {
   if ((x | y | z | xy | xz | yz | xyz) >= 0) {
-    if (fy <= 1 && fz <= 1) return fc3_small(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+    if (fy <= 1 && fz <= 1) {
+      const int nx = x + xy + xz + xyz;
+      if (fc3_small_inline_ok(fx, nx)) return fc3_small_inline(fx, fy, fz, nx, y, z, xy, xz, yz, xyz);
+      return fc3_small(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
+    }
     if (fy <= 2 && fz <= 2) {
       switch (fy * 3 + fz) {
         case 2: return fc3_fixed<0, 2>(fx, x, y, z, xy, xz, yz, xyz, fl);
@@ -378,12 +423,19 @@ __device__ __forceinline__ u128 fc3_xyz(int fx, int fy, int fz, int x, int y, in
 
 __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, uint32_t& fl)
 {
+  // X = the core vertex with the most fringes (ties: u, then v), Y and Z the others in
+  // core order; the arguments are selected first so that fc3_xyz is expanded once
   const int fu = fr[iu], fv = fr[iv], fw = fr[iw];
-  if (fu >= fv && fu >= fw)
-    return fc3_xyz(fu, fv, fw, venn[iu], venn[iv], venn[iw], venn[iuv], venn[iuw], venn[ivw], venn[iuvw], fl);
-  if (fv >= fw)
-    return fc3_xyz(fv, fu, fw, venn[iv], venn[iu], venn[iw], venn[iuv], venn[ivw], venn[iuw], venn[iuvw], fl);
-  return fc3_xyz(fw, fu, fv, venn[iw], venn[iu], venn[iv], venn[iuw], venn[ivw], venn[iuv], venn[iuvw], fl);
+  const int U = venn[iu], V = venn[iv], W = venn[iw], UV = venn[iuv], UW = venn[iuw], VW = venn[ivw];
+  int fx, fy, fz, x, y, z, xy, xz, yz;
+  if (fu >= fv && fu >= fw) {
+    fx = fu; fy = fv; fz = fw; x = U; y = V; z = W; xy = UV; xz = UW; yz = VW;
+  } else if (fv >= fw) {
+    fx = fv; fy = fu; fz = fw; x = V; y = U; z = W; xy = UV; xz = VW; yz = UW;
+  } else {
+    fx = fw; fy = fu; fz = fv; x = W; y = U; z = V; xy = UW; xz = VW; yz = UV;
+  }
+  return fc3_xyz(fx, fy, fz, x, y, z, xy, xz, yz, venn[iuvw], fl);
 }
 
 __device__ __forceinline__ bool fc3_is_singleton(const int* fr) { return (fr[iuv] | fr[iuw] | fr[ivw] | fr[iuvw]) == 0; }
