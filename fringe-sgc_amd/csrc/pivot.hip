@@ -46,7 +46,12 @@ constexpr int kMaxPivotJobs = 8;
 // in the global per-workgroup array D. (Launches with c3 keep their 128 KB of windows:
 // 8184 hubs in the remaining LDS measured 2% slower than none, the lanes of a wave
 // diverging between the two kinds of counter.)
-constexpr int kHubSlots = 40928;
+constexpr int kHubSlots = 39808;
+// Phase B defers centres whose walk spans several windows (kECap positions with c3,
+// kWalkChunk without) to a task list shared by the workgroup's waves, up to kHeavyCap
+// per item; the hub counters above leave LDS room for that list (4 KB)
+constexpr int kHeavyCap = 512;
+constexpr int kWalkChunk = 2048;
 
 struct PivotItem {
   int32_t w;   // pivot
@@ -244,7 +249,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
 {
   __shared__ unsigned s_e[kPivWaves][kECap];
   __shared__ unsigned s_hub[kHub > 0 ? kHub : 1];
-  __shared__ int s_item, s_next;
+  __shared__ int s_item, s_next, s_nheavy, s_task;
+  __shared__ int s_heavy[kHeavyCap], s_hwin[kHeavyCap];
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
   unsigned* __restrict__ D = dbuf + (long long)blockIdx.x * dstride;
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     for (int i = threadIdx.x; i < kHub; i += kPivWaves * kWave) s_hub[i] = 0u;
 
   for (;;) {
-    if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; }
+    if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; s_nheavy = 0; s_task = 0; }
     __syncthreads();
     const int it = s_item;
     if (it >= nitems) break;
@@ -311,7 +317,137 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     }
     __syncthreads();
 
-    // Phase B: centres, one per wave at a time
+    // one window [q_lo, q_hi) of centre u's walk past w: the c3 scatter into this
+    // wave's LDS counters (when c3on), then the walk
+    auto window = [&](int u, int ub, int du, int tuw, bool c3on, long long tb, long long te, int q_lo, int q_hi) {
+      unsigned* __restrict__ E = s_e[wave];
+      const bool last = q_hi == du;
+      if (c3on) {
+        for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // scatter: for x in T(u, w), for y in T(u, x) with q_lo <= y < q_hi: E[y - q_lo]++
+        for (long long t0 = tb; t0 < te; t0 += kWave) {
+          const long long t = t0 + lane;
+          int xq = -1;
+          if (t < te) xq = (int)tl[t];
+          // each lane walks its own list; long lists are shared by the wave
+          long long lb = 0, le = 0;
+          if (xq >= 0) {
+            const long long b0 = tl_off[ub + xq], e0 = tl_off[ub + xq + 1];
+            long long lo = b0, hi = e0;  // first entry >= q_lo
+            while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
+            lb = lo;
+            le = e0;
+            if (!last) {  // first entry >= q_hi
+              hi = e0;
+              while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
+              le = lo;
+            }
+            bytes += 16 + sizeof(TL) + (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) +
+                     (le - lb) * sizeof(TL);
+          }
+          const bool big = (le - lb) > 64;
+          if (!big) {
+            long long k = lb;
+            for (; k + 4 <= le; k += 4) {
+              const int a0 = (int)tl[k], a1 = (int)tl[k + 1], a2 = (int)tl[k + 2], a3 = (int)tl[k + 3];
+              atomicAdd(&E[a0 - q_lo], 1u);
+              atomicAdd(&E[a1 - q_lo], 1u);
+              atomicAdd(&E[a2 - q_lo], 1u);
+              atomicAdd(&E[a3 - q_lo], 1u);
+            }
+            for (; k < le; k++) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
+          }
+          ull bm = __ballot(big);
+          while (bm) {
+            const int who = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            const long long B = bcast(lb, who), Ee = bcast(le, who);
+            long long k = B + lane;
+            if constexpr (sizeof(TL) == 2) {
+              // 16-bit lists: four positions per 8-byte load, four loads in flight per
+              // lane. The head up to the first multiple of 4 (8-byte aligned, as tl
+              // itself is) and the tail past the last full group go one per lane.
+              const long long a4 = min(Ee, (B + 3) & ~3ll);
+              if (B + lane < a4) atomicAdd(&E[(int)tl[B + lane] - q_lo], 1u);
+              const long long n4 = (Ee - a4) >> 2;
+              const ushort4* __restrict__ v4 = reinterpret_cast<const ushort4*>(tl + a4);
+              long long g4 = lane;
+              for (; g4 + 3 * kWave < n4; g4 += 4 * kWave) {
+                ushort4 r[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) r[i] = v4[g4 + i * kWave];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                  atomicAdd(&E[(int)r[i].x - q_lo], 1u);
+                  atomicAdd(&E[(int)r[i].y - q_lo], 1u);
+                  atomicAdd(&E[(int)r[i].z - q_lo], 1u);
+                  atomicAdd(&E[(int)r[i].w - q_lo], 1u);
+                }
+              }
+              for (; g4 < n4; g4 += kWave) {
+                const ushort4 r = v4[g4];
+                atomicAdd(&E[(int)r.x - q_lo], 1u);
+                atomicAdd(&E[(int)r.y - q_lo], 1u);
+                atomicAdd(&E[(int)r.z - q_lo], 1u);
+                atomicAdd(&E[(int)r.w - q_lo], 1u);
+              }
+              k = a4 + (n4 << 2) + lane;  // the tail, below
+            } else {
+              // eight independent loads in flight per lane before their increments
+              for (; k + 7 * kWave < Ee; k += 8 * kWave) {
+                int a[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
+#pragma unroll
+                for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
+              }
+            }
+            for (; k < Ee; k += kWave) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      // walk the window
+      for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
+        const int q = q0 + lane;
+        if (q < q_hi) {
+          const int h = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
+          unsigned dv_;
+          if (kHub > 0 && h >= 0) {
+            dv_ = s_hub[h];
+          } else {
+            const unsigned draw = D[~h];
+            dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
+            bytes += 4;
+          }
+          const int tuv = tsup[ub + q], dv = adjdeg[ub + q];
+          const bool tri = dv_ & 1u;
+          const int cvw = (int)(dv_ >> 1);
+          const int c3 = c3on ? (int)E[q - q_lo] : 0;
+          bytes += 12;
+#pragma unroll
+          for (int j = 0; j < NJ; j++) {
+            if (j < njobs) {
+              u128 val;
+              if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
+                acc[j].add(val);
+                acc[j].matches++;
+              }
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    };
+
+    // Phase B: centres, one per wave at a time. A centre whose walk past w spans more
+    // than one window is deferred to phase B2 (up to kHeavyCap of them per item), where
+    // its windows are shared by all waves: a hub centre would otherwise keep one wave
+    // busy while the others wait at the item's closing barrier.
     for (;;) {
       int c = 0;
       if (lane == 0) c = atomicAdd(&s_next, 1);
@@ -327,137 +463,70 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       if (jobs[0].smod > 1) in_slice = (u % jobs[0].smod) == jobs[0].srem;
       if (!in_slice) continue;
       if (lane == 0) bytes += 16 + 4 * (32 - __clz(du + 1));
-      unsigned* __restrict__ E = s_e[wave];
       const bool c3on = need_c3 && tuw > 0;
+      const int cstep = c3on ? kECap : kWalkChunk;
+      const int nwin = (du - (rq + 1) + cstep - 1) / cstep;
+      if (nwin > 1) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(&s_nheavy, 1);
+        slot = bcast(slot, 0);
+        if (slot < kHeavyCap) {
+          if (lane == 0) { s_heavy[slot] = p; s_hwin[slot] = nwin; }
+          continue;
+        }
+      }
       const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
       // the walk past w in windows of kECap positions, each with its c3 counters in LDS
       // (without c3 the whole walk is one window)
       const int wstep = c3on ? kECap : du;
-      for (int q_lo = rq + 1; q_lo < du; q_lo += wstep) {
-        const int q_hi = min(du, q_lo + wstep);
-        const bool last = q_hi == du;
-        if (c3on) {
-          for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          // scatter: for x in T(u, w), for y in T(u, x) with q_lo <= y < q_hi: E[y - q_lo]++
-          for (long long t0 = tb; t0 < te; t0 += kWave) {
-            const long long t = t0 + lane;
-            int xq = -1;
-            if (t < te) xq = (int)tl[t];
-            // each lane walks its own list; long lists are shared by the wave
-            long long lb = 0, le = 0;
-            if (xq >= 0) {
-              const long long b0 = tl_off[ub + xq], e0 = tl_off[ub + xq + 1];
-              long long lo = b0, hi = e0;  // first entry >= q_lo
-              while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
-              lb = lo;
-              le = e0;
-              if (!last) {  // first entry >= q_hi
-                hi = e0;
-                while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
-                le = lo;
-              }
-              bytes += 16 + sizeof(TL) + (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) +
-                       (le - lb) * sizeof(TL);
-            }
-            const bool big = (le - lb) > 64;
-            if (!big) {
-              long long k = lb;
-              for (; k + 4 <= le; k += 4) {
-                const int a0 = (int)tl[k], a1 = (int)tl[k + 1], a2 = (int)tl[k + 2], a3 = (int)tl[k + 3];
-                atomicAdd(&E[a0 - q_lo], 1u);
-                atomicAdd(&E[a1 - q_lo], 1u);
-                atomicAdd(&E[a2 - q_lo], 1u);
-                atomicAdd(&E[a3 - q_lo], 1u);
-              }
-              for (; k < le; k++) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
-            }
-            ull bm = __ballot(big);
-            while (bm) {
-              const int who = __builtin_ctzll(bm);
-              bm &= bm - 1;
-              const long long B = bcast(lb, who), Ee = bcast(le, who);
-              long long k = B + lane;
-              if constexpr (sizeof(TL) == 2) {
-                // 16-bit lists: four positions per 8-byte load, four loads in flight per
-                // lane. The head up to the first multiple of 4 (8-byte aligned, as tl
-                // itself is) and the tail past the last full group go one per lane.
-                const long long a4 = min(Ee, (B + 3) & ~3ll);
-                if (B + lane < a4) atomicAdd(&E[(int)tl[B + lane] - q_lo], 1u);
-                const long long n4 = (Ee - a4) >> 2;
-                const ushort4* __restrict__ v4 = reinterpret_cast<const ushort4*>(tl + a4);
-                long long g4 = lane;
-                for (; g4 + 3 * kWave < n4; g4 += 4 * kWave) {
-                  ushort4 r[4];
+      for (int q_lo = rq + 1; q_lo < du; q_lo += wstep) window(u, ub, du, tuw, c3on, tb, te, q_lo, min(du, q_lo + wstep));
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Phase B2: the deferred centres' windows as one task list (inclusive prefix of
+    // their window counts in s_hwin), taken one window per wave at a time
+    const int nh = min(s_nheavy, kHeavyCap);
+    if (nh > 0) {
+      if (wave == 0) {
+        int carry = 0;
+        for (int b0 = 0; b0 < nh; b0 += kWave) {
+          const int h = b0 + lane;
+          int v = h < nh ? s_hwin[h] : 0;
 #pragma unroll
-                  for (int i = 0; i < 4; i++) r[i] = v4[g4 + i * kWave];
-#pragma unroll
-                  for (int i = 0; i < 4; i++) {
-                    atomicAdd(&E[(int)r[i].x - q_lo], 1u);
-                    atomicAdd(&E[(int)r[i].y - q_lo], 1u);
-                    atomicAdd(&E[(int)r[i].z - q_lo], 1u);
-                    atomicAdd(&E[(int)r[i].w - q_lo], 1u);
-                  }
-                }
-                for (; g4 < n4; g4 += kWave) {
-                  const ushort4 r = v4[g4];
-                  atomicAdd(&E[(int)r.x - q_lo], 1u);
-                  atomicAdd(&E[(int)r.y - q_lo], 1u);
-                  atomicAdd(&E[(int)r.z - q_lo], 1u);
-                  atomicAdd(&E[(int)r.w - q_lo], 1u);
-                }
-                k = a4 + (n4 << 2) + lane;  // the tail, below
-              } else {
-                // eight independent loads in flight per lane before their increments
-                for (; k + 7 * kWave < Ee; k += 8 * kWave) {
-                  int a[8];
-#pragma unroll
-                  for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
-#pragma unroll
-                  for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
-                }
-              }
-              for (; k < Ee; k += kWave) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
-            }
+          for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(v, o, kWave);
+            if (lane >= o) v += t;
           }
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          __builtin_amdgcn_wave_barrier();
+          if (h < nh) s_hwin[h] = v + carry;
+          carry += __shfl(v, kWave - 1, kWave);
         }
-        // walk the window
-        for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
-          const int q = q0 + lane;
-          if (q < q_hi) {
-            const int h = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
-            unsigned dv_;
-            if (kHub > 0 && h >= 0) {
-              dv_ = s_hub[h];
-            } else {
-              const unsigned draw = D[~h];
-              dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
-              bytes += 4;
-            }
-            const int tuv = tsup[ub + q], dv = adjdeg[ub + q];
-            const bool tri = dv_ & 1u;
-            const int cvw = (int)(dv_ >> 1);
-            const int c3 = c3on ? (int)E[q - q_lo] : 0;
-            bytes += 12;
-#pragma unroll
-            for (int j = 0; j < NJ; j++) {
-              if (j < njobs) {
-                u128 val;
-                if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
-                  acc[j].add(val);
-                  acc[j].matches++;
-                }
-              }
-            }
-          }
+      }
+      __syncthreads();
+      const int ntask = s_hwin[nh - 1];
+      for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&s_task, 1);
+        t = bcast(t, 0);
+        if (t >= ntask) break;
+        int lo = 0, hi = nh - 1;  // first h with s_hwin[h] > t
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_hwin[mid] > t) hi = mid; else lo = mid + 1;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int k = t - (lo > 0 ? s_hwin[lo - 1] : 0);
+        const int p = s_heavy[lo];
+        const int u = adj[wb + p];
+        const int ub = g.beg(u), ue = g.end(u), du = ue - ub;
+        const int tuw = tsup[wb + p];
+        const int rq = lower_bound_pos(adj, ub, ue, w) - ub;
+        const bool c3on = need_c3 && tuw > 0;
+        const int cstep = c3on ? kECap : kWalkChunk;
+        const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
+        const int q_lo = rq + 1 + k * cstep;
+        window(u, ub, du, tuw, c3on, tb, te, q_lo, min(du, q_lo + cstep));
         __builtin_amdgcn_wave_barrier();
       }
-      __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
     if (kHub > 0)
