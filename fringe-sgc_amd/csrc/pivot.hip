@@ -52,6 +52,8 @@ constexpr int kHubSlots = 39808;
 // per item; the hub counters above leave LDS room for that list (4 KB)
 constexpr int kHeavyCap = 512;
 constexpr int kWalkChunk = 2048;
+// phase A defers an x whose list past w is longer than kAChunk positions the same way
+constexpr int kAChunk = 2048;
 
 struct PivotItem {
   int32_t w;   // pivot
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
 {
   __shared__ unsigned s_e[kPivWaves][kECap];
   __shared__ unsigned s_hub[kHub > 0 ? kHub : 1];
-  __shared__ int s_item, s_next, s_nheavy, s_task;
+  __shared__ int s_item, s_next, s_nheavy, s_task, s_anext, s_anheavy, s_atask;
   __shared__ int s_heavy[kHeavyCap], s_hwin[kHeavyCap];
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -271,7 +273,11 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     for (int i = threadIdx.x; i < kHub; i += kPivWaves * kWave) s_hub[i] = 0u;
 
   for (;;) {
-    if (threadIdx.x == 0) { s_item = atomicAdd(head, 1); s_next = 0; s_nheavy = 0; s_task = 0; }
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(head, 1);
+      s_next = s_nheavy = s_task = 0;
+      s_anext = s_anheavy = s_atask = 0;
+    }
     __syncthreads();
     const int it = s_item;
     if (it >= nitems) break;
@@ -285,9 +291,58 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     }
     const unsigned e0 = epoch << 24;
 
+    // Deferred heavy work of phases A and B: entries s_heavy[h] with their task counts,
+    // turned by wave 0 into an inclusive prefix in s_hwin (nh <= kHeavyCap) ...
+    auto heavy_prefix = [&](int nh) {
+      if (wave == 0) {
+        int carry = 0;
+        for (int b0 = 0; b0 < nh; b0 += kWave) {
+          const int h = b0 + lane;
+          int v = h < nh ? s_hwin[h] : 0;
+#pragma unroll
+          for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(v, o, kWave);
+            if (lane >= o) v += t;
+          }
+          if (h < nh) s_hwin[h] = v + carry;
+          carry += __shfl(v, kWave - 1, kWave);
+        }
+      }
+    };
+    // ... and task t mapped to (entry h, task k of that entry)
+    auto heavy_task = [&](int t, int nh, int& h, int& k) {
+      int lo = 0, hi = nh - 1;  // first h with s_hwin[h] > t
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_hwin[mid] > t) hi = mid; else lo = mid + 1;
+      }
+      h = lo;
+      k = t - (lo > 0 ? s_hwin[lo - 1] : 0);
+    };
+
     // Phase A: D[y] += 2 for every path w - x - y with y > w; D[x] |= 1 for x in N(w).
     // With hub slots, adjhub[e] names the LDS counter of a hub (>= 0) or ~y otherwise.
-    for (int i = wb + wave; i < we; i += kPivWaves) {
+    // Waves take the x one at a time; an x with more than kAChunk positions past w is
+    // deferred (phase A2) and its chunks shared by all waves.
+    auto scatter2 = [&](int k0, int k1) {
+      for (int k = k0 + lane; k < k1; k += kWave) {
+        const int h = kHub > 0 ? adjhub[k] : ~adj[k];
+        if (kHub > 0 && h >= 0) {
+          atomicAdd(&s_hub[h], 2u);
+          bytes += 4;
+        } else {
+          atomicMax(&D[~h], e0);
+          atomicAdd(&D[~h], 2u);
+          bytes += 20;
+        }
+      }
+    };
+    for (;;) {
+      int c = 0;
+      if (lane == 0) c = atomicAdd(&s_anext, 1);
+      c = bcast(c, 0);
+      const int i = wb + c;
+      if (i >= we) break;
       const int x = adj[i];
       const int xb = g.beg(x), xe = g.end(x);
       if (lane == 0 && x > w) {
@@ -303,19 +358,40 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
       if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1));
-      for (int k = s + lane; k < xe; k += kWave) {
-        const int h = kHub > 0 ? adjhub[k] : ~adj[k];
-        if (kHub > 0 && h >= 0) {
-          atomicAdd(&s_hub[h], 2u);
-          bytes += 4;
-        } else {
-          atomicMax(&D[~h], e0);
-          atomicAdd(&D[~h], 2u);
-          bytes += 20;
+      const int nch = (xe - s + kAChunk - 1) / kAChunk;
+      if (nch > 1) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(&s_anheavy, 1);
+        slot = bcast(slot, 0);
+        if (slot < kHeavyCap) {
+          if (lane == 0) { s_heavy[slot] = i; s_hwin[slot] = nch; }
+          continue;
         }
       }
+      scatter2(s, xe);
     }
     __syncthreads();
+    {
+      const int nh = min(s_anheavy, kHeavyCap);
+      if (nh > 0) {
+        heavy_prefix(nh);
+        __syncthreads();
+        const int ntask = s_hwin[nh - 1];
+        for (;;) {
+          int t = 0;
+          if (lane == 0) t = atomicAdd(&s_atask, 1);
+          t = bcast(t, 0);
+          if (t >= ntask) break;
+          int h, k;
+          heavy_task(t, nh, h, k);
+          const int x = adj[s_heavy[h]];
+          const int xb = g.beg(x), xe = g.end(x);
+          const int s = lower_bound_pos(adj, xb, xe, w + 1) + k * kAChunk;
+          scatter2(s, min(xe, s + kAChunk));
+        }
+        __syncthreads();
+      }
+    }
 
     // one window [q_lo, q_hi) of centre u's walk past w: the c3 scatter into this
     // wave's LDS counters (when c3on), then the walk
@@ -484,24 +560,11 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     }
     __syncthreads();
 
-    // Phase B2: the deferred centres' windows as one task list (inclusive prefix of
-    // their window counts in s_hwin), taken one window per wave at a time
+    // Phase B2: the deferred centres' windows as one task list, taken one window per
+    // wave at a time
     const int nh = min(s_nheavy, kHeavyCap);
     if (nh > 0) {
-      if (wave == 0) {
-        int carry = 0;
-        for (int b0 = 0; b0 < nh; b0 += kWave) {
-          const int h = b0 + lane;
-          int v = h < nh ? s_hwin[h] : 0;
-#pragma unroll
-          for (int o = 1; o < kWave; o <<= 1) {
-            const int t = __shfl_up(v, o, kWave);
-            if (lane >= o) v += t;
-          }
-          if (h < nh) s_hwin[h] = v + carry;
-          carry += __shfl(v, kWave - 1, kWave);
-        }
-      }
+      heavy_prefix(nh);
       __syncthreads();
       const int ntask = s_hwin[nh - 1];
       for (;;) {
@@ -509,13 +572,9 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         if (lane == 0) t = atomicAdd(&s_task, 1);
         t = bcast(t, 0);
         if (t >= ntask) break;
-        int lo = 0, hi = nh - 1;  // first h with s_hwin[h] > t
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_hwin[mid] > t) hi = mid; else lo = mid + 1;
-        }
-        const int k = t - (lo > 0 ? s_hwin[lo - 1] : 0);
-        const int p = s_heavy[lo];
+        int hh, k;
+        heavy_task(t, nh, hh, k);
+        const int p = s_heavy[hh];
         const int u = adj[wb + p];
         const int ub = g.beg(u), ue = g.end(u), du = ue - ub;
         const int tuw = tsup[wb + p];
