@@ -154,7 +154,7 @@ def test_triangle_rank_schedule_matches_per_root(gpu):
 
 
 def test_wedge_pivot_hub_and_global_counters(gpu):
-    """Graphs with more vertices than the pivot kernel has hub slots (40928). Launches
+    """Graphs with more vertices than the pivot kernel has hub slots (39808). Launches
     without c3 (the closed-form wedge kinds) then keep the 2-hop counters of the
     highest-degree vertices in LDS and the others in global memory; launches with c3,
     and the 4- and 8-wave shapes, keep all of them in global memory. Each shape, and a
