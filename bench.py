@@ -41,8 +41,8 @@ WORKLOADS = {
         "name": "R-MAT s20 ef16 (0.57,0.19,0.19) seed 7; wedge core + 3 centre / 1+1 endpoint fringes",
         "cpu_slice": (4096,),
         "kernel": "k_wedge_pivot",
-        # counted on the graph renumbered by ascending degree (3.45 s against 4.27 s on
-        # the generator's ids and 3.83 s on the reference's descending order)
+        # counted on the graph renumbered by ascending degree (2.24 s against 3.47 s on
+        # the generator's ids and 3.31 s on the reference's descending order)
         "relabel": "asc",
     },
     "c1": {
