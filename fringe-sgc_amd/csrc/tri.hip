@@ -100,6 +100,9 @@ __device__ __forceinline__ bool eval_triangle(const JobDesc* __restrict__ J, int
   return true;
 }
 
+// T(x,z) lists longer than this are counted by the whole wave (see k_triangle_rank)
+constexpr int kCoopLen = 64;
+
 template <int NJ, bool kFull>
 __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
                                                              const int* __restrict__ tsup,
@@ -157,9 +160,15 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      // lanes over z in T(x, y) ranked above y
+      // lanes over z in T(x, y) ranked above y. Each lane counts c3 = |T(x,y) & T(x,z)|
+      // over its own list T(x,z), except lists longer than kCoopLen in launches that need
+      // no c3lt: those are counted by the whole wave, one list at a time, so that one
+      // hub list does not keep the other lanes waiting.
       for (long long k0 = yb; k0 < ye; k0 += kWave) {
         const long long k = k0 + lane;
+        bool act = false;
+        int da = 0, db = 0, dc = 0, tab = 0, tac = 0, tbc = 0, wpos = 0;
+        long long zb = 0, ze = 0;
         if (k < ye) {
           const int pz = tl[k];
           const int z = adj[xb + pz];
@@ -170,41 +179,61 @@ __global__ __launch_bounds__(kTriBlock) void k_triangle_rank(const int* __restri
             const int tyz = tsup[yb2 + tl[yxb + (k - yb)]];
             bytes += 12;
             // reference roles by id: u > v > w
-            int a = x, da = dx, b = y, db = dy, c = z, dc = dz;
-            int tab = txy, tac = txz, tbc = tyz;
+            int a = x, b = y, c = z;
+            da = dx; db = dy; dc = dz;
+            tab = txy; tac = txz; tbc = tyz;
             // sort (a, b, c) descending by id, carrying degrees and supports
             if (a < b) { int t = a; a = b; b = t; t = da; da = db; db = t; t = tac; tac = tbc; tbc = t; }
             if (b < c) { int t = b; b = c; c = t; t = db; db = dc; dc = t; t = tab; tab = tac; tac = t; }
             if (a < b) { int t = a; a = b; b = t; t = da; da = db; db = t; t = tac; tac = tbc; tbc = t; }
             const int u = a;
-            if (smod > 1 && (u % smod) != srem) continue;
-            int c3 = 0, c3lt = 0;
-            u128 wq = 0;
-            uint32_t fl0 = 0;
-            if (need_c3) {
-              const long long zb = tl_off[xb + pz], ze = tl_off[xb + pz + 1];
-              const int wpos = need_lt ? lbound(adj, xb, xb + dx, c) - xb : 0;  // positions below w = c
+            act = !(smod > 1 && (u % smod) != srem);
+            if (act && need_c3) {
+              zb = tl_off[xb + pz];
+              ze = tl_off[xb + pz + 1];
+              wpos = need_lt ? lbound(adj, xb, xb + dx, c) - xb : 0;  // positions below w = c
               bytes += 16 + (need_lt ? 4 * (32 - __clz(dx + 1)) : 0) + (ze - zb) * (lds ? 4 : 5);
-              for (long long q = zb; q < ze; q++) {
-                const int pq = tl[q];
-                if (M[pq]) {
-                  c3++;
-                  if (need_lt && pq < wpos) {
-                    c3lt++;
-                    if (tails) wq = addx(wq, ncr(adjdeg[xb + pq] - 3, tails, fl0), fl0);
-                  }
+            }
+          }
+        }
+        int c3 = 0, c3lt = 0;
+        u128 wq = 0;
+        uint32_t fl0 = 0;
+        if (need_c3) {
+          const bool big = kFull && act && !need_lt && (ze - zb) > kCoopLen;
+          if (act && !big) {
+            for (long long q = zb; q < ze; q++) {
+              const int pq = tl[q];
+              if (M[pq]) {
+                c3++;
+                if (need_lt && pq < wpos) {
+                  c3lt++;
+                  if (tails) wq = addx(wq, ncr(adjdeg[xb + pq] - 3, tails, fl0), fl0);
                 }
               }
             }
+          }
+          ull bm = __ballot(big);
+          while (bm) {
+            const int who = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            const long long B = bcast(zb, who), Ee = bcast(ze, who);
+            int cnt = 0;
+            for (long long q = B + lane; q < Ee; q += kWave) cnt += M[tl[q]] ? 1 : 0;
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-              if (j < njobs) {
-                u128 val;
-                if (eval_triangle<kFull>(jobs + j, da, db, dc, tab, tac, tbc, c3, c3lt, wq, val, acc[j].fl)) {
-                  acc[j].add(val);
-                  acc[j].matches++;
-                  acc[j].fl |= fl0;
-                }
+            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
+            if (lane == who) c3 = cnt;
+          }
+        }
+        if (act) {
+#pragma unroll
+          for (int j = 0; j < NJ; j++) {
+            if (j < njobs) {
+              u128 val;
+              if (eval_triangle<kFull>(jobs + j, da, db, dc, tab, tac, tbc, c3, c3lt, wq, val, acc[j].fl)) {
+                acc[j].add(val);
+                acc[j].matches++;
+                acc[j].fl |= fl0;
               }
             }
           }
