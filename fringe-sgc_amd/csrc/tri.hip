@@ -103,10 +103,11 @@ __device__ __forceinline__ bool eval_triangle(const JobDesc* __restrict__ J, int
 // T(x,z) lists longer than this are counted by the whole wave (see k_triangle_rank)
 constexpr int kCoopLen = 64;
 
-// Three waves per SIMD: the full-fc3 routine (fc3_peeled) would otherwise take 248
-// VGPRs and hold the kernel to two; at three it spills a little
+// Four waves per SIMD: the full-fc3 routine (fc3_peeled) would otherwise take 248
+// VGPRs and hold the kernel to two; at four it spills (C4: 1.43 s against 1.55 s at
+// three waves and 2.06 s at two)
 template <int NJ, bool kFull>
-__global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
+__global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
                                                              const int* __restrict__ tsup,
                                                              const int* __restrict__ adjdeg,
                                                              const long long* __restrict__ tl_off,
