@@ -16,7 +16,10 @@
 //     indexed by position in N(u), one window of kECap positions at a time (hub
 //     centres take several windows; each x's list is cut to the window by two
 //     binary searches);
-//   * a wave then walks the window and evaluates every wedge job per lane.
+//   * a wave then walks the window and evaluates every wedge job per lane;
+//   * within a workgroup the waves share the work of one pivot through LDS task
+//     lists: neighbours x with long lists (2-hop scatter) and centres u with several
+//     windows (c3 scatter and walk) are cut into chunks that any wave takes.
 // The sum over wedges is the reference's, with the same per-wedge filters, formulas
 // and counted matches; only the order of evaluation differs.
 #include <hip/hip_runtime.h>
