@@ -90,15 +90,20 @@ int fsgc_graph_from_csr(int32_t nodes, int32_t edges, const int32_t* nindex, con
   if (!out || !nindex || (edges > 0 && !nlist)) return fail(-22, "null argument");
   if (nodes < 1 || edges < 0) return fail(-22, "ERROR: node or edge count too low");
   if (nindex[0] != 0 || nindex[nodes] != edges) return fail(-22, "nindex does not span nlist");
-  int rc = ensure_init();
-  if (rc) return rc;
   fsgc::HostGraph hg;
   hg.n = nodes;
   hg.m = edges;
   hg.nindex.assign(nindex, nindex + nodes + 1);
   if (edges) hg.nlist.assign(nlist, nlist + edges);
-  fsgc_graph* g = new fsgc_graph();
   std::string err;
+  // no kernel or host index build may see a malformed graph: a non-monotone nindex or
+  // an id out of range would index past the device arrays (checked before the GPU is
+  // touched)
+  int rc = fsgc::validate_csr(hg, &err);
+  if (rc) return fail(rc, err);
+  rc = ensure_init();
+  if (rc) return rc;
+  fsgc_graph* g = new fsgc_graph();
   rc = fsgc::device_graph_upload(hg, &g->dg, &err);
   if (rc) { fsgc::device_graph_free(&g->dg); delete g; return fail(rc, err); }
   *out = g;
@@ -194,6 +199,18 @@ int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* 
     out[i].index_ms = res[i].index_ms;
   }
   if (batch_ms) *batch_ms = ms;
+  return 0;
+}
+
+int fsgc_plan_schedule(const uint8_t mo[FSGC_MO_BYTES], int32_t maxdeg, int32_t* schedule, int32_t* motif_class)
+{
+  if (!mo || !schedule) return fail(-22, "null argument");
+  fsgc::Prepared p;
+  std::string err;
+  const int rc = fsgc::prepare(mo, nullptr, &p, &err);
+  if (rc) return fail(rc, err);
+  *schedule = p.trivial ? -1 : (int32_t)fsgc::schedule_of(p, true, maxdeg);
+  if (motif_class) *motif_class = p.job.cls;
   return 0;
 }
 

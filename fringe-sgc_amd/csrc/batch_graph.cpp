@@ -73,7 +73,7 @@ int fsgc_batch_create(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune*
   double ms = 0;
   rc = fsgc::run_prepared(g, b->ps, tune, &res, &ms, b->stream, &err);
   if (rc) return fail(rc, err);
-  b->groups = fsgc::make_groups(b->ps, true);
+  b->groups = fsgc::make_groups(b->ps, true, g->dg.maxdeg);
   for (const fsgc::Group& gr : b->groups) {
     fsgc::LaunchPlan lp;
     rc = fsgc::plan_group(g, b->ps, gr, tune, 1, b->stream, &lp, &err);

@@ -100,10 +100,12 @@ void finish(const Prepared& p, const JobResult& r, int32_t n, int64_t m, fsgc_re
 // The pivot schedule for wedge cores, the rank schedule for triangle cores and the
 // triangle kinds of the edge core, one edge pass for the other edge kinds and the
 // stars, the per-root persistent kernel otherwise (or for everything when !fast).
-Schedule schedule_of(const Prepared& p, bool fast)
+// Wedge cores on a graph whose maximum degree exceeds kPivotMaxDegree take the
+// per-root kernel: the pivot kernel's 24-bit 2-hop counters could not hold a codegree.
+Schedule schedule_of(const Prepared& p, bool fast, int32_t maxdeg)
 {
   if (!fast) return kPersistent;
-  if (p.job.cls == kWedge) return kPivot;
+  if (p.job.cls == kWedge) return maxdeg <= kPivotMaxDegree ? kPivot : kPersistent;
   if (triangle_schedule_ok(p.job)) return kRank;
   if (edgepass_schedule_ok(p.job)) return kEdgePass;
   return kPersistent;
@@ -161,19 +163,19 @@ int group_tier(const JobDesc& j)
 
 }  // namespace
 
-std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast)
+std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast, int32_t maxdeg)
 {
   std::vector<Group> out;
   Group perroot{kPersistent, {}};
   for (size_t i = 0; i < ps.size(); i++)
-    if (!ps[i].trivial && schedule_of(ps[i], fast) == kPersistent) perroot.idx.push_back(i);
+    if (!ps[i].trivial && schedule_of(ps[i], fast, maxdeg) == kPersistent) perroot.idx.push_back(i);
   if (!perroot.idx.empty()) out.push_back(perroot);
   if (!fast) return out;
   // each fast-schedule launch takes up to 8 motifs that share one enumeration
   for (int sched : {kPivot, kRank, kEdgePass}) {
     std::vector<Group> groups;
     for (size_t i = 0; i < ps.size(); i++) {
-      if (ps[i].trivial || schedule_of(ps[i], fast) != sched) continue;
+      if (ps[i].trivial || schedule_of(ps[i], fast, maxdeg) != sched) continue;
       // tailed 4-cliques with different tail counts go to separate launches
       const int key = (ps[i].job.cls == kTriangle && ps[i].job.kind == kTriTailedClique) ? ps[i].job.p[3] : -1;
       const int tier = group_tier(ps[i].job);
@@ -238,7 +240,7 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
       jobs.push_back(hj);
       graphs.push_back(&mgs[i]);
     }
-    if (!p.trivial && schedule_of(p, fast) == kPersistent) {
+    if (!p.trivial && schedule_of(p, fast, g->dg.maxdeg) == kPersistent) {
       data_job[i] = (int)jobs.size();
       jobs.push_back(p.job);
       graphs.push_back(&g->dg);
@@ -255,7 +257,7 @@ int run_prepared(fsgc_graph* g, std::vector<Prepared>& ps, const fsgc_tune* tune
   if (log_on() && !jobs.empty()) fprintf(stderr, "[fsgc] per-root launch: %zu jobs, %.1f ms\n", jobs.size(), ms);
   std::vector<JobResult> sres(ps.size());
   if (!rc) {
-    for (const Group& gr : make_groups(ps, fast)) {
+    for (const Group& gr : make_groups(ps, fast, g->dg.maxdeg)) {
       if (gr.sched == kPersistent) continue;  // ran above, with the automorphism jobs
       LaunchPlan lp;
       std::vector<JobResult> r;

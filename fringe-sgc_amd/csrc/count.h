@@ -52,7 +52,12 @@ int current_device();
 void set_error(const std::string& msg);  // text returned by fsgc_last_error on this thread
 bool log_on();
 int prepare(const uint8_t* mob, const fsgc_tune* tune, Prepared* p, std::string* err);
-Schedule schedule_of(const Prepared& p, bool fast);
+// The pivot kernel keeps 2 * c(w, y) + adjacency of a pivot w and a vertex y in the low
+// 24 bits of a counter (the top 8 hold an epoch); c(w, y) <= maxdeg, so the pivot
+// schedule takes graphs of maximum degree up to 2^23 - 1 only.
+constexpr int32_t kPivotMaxDegree = (1 << 23) - 1;
+// The kernel that counts a motif on a graph of maximum degree `maxdeg`.
+Schedule schedule_of(const Prepared& p, bool fast, int32_t maxdeg);
 // Launch shape for one schedule: the caller's non-zero fields, else the graph's tuned
 // values, else the defaults.
 Tune tune_for(fsgc_graph* g, const fsgc_tune* t, int sched);
@@ -65,7 +70,7 @@ struct Group {
   int sched;
   std::vector<size_t> idx;  // indices into the prepared motifs
 };
-std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast);
+std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast, int32_t maxdeg);
 int plan_group(fsgc_graph* g, const std::vector<Prepared>& ps, const Group& gr, const fsgc_tune* tune, int sample,
                void* stream, LaunchPlan* lp, std::string* err);
 

@@ -1,6 +1,8 @@
 // ECLgraph .egr reader/writer and matching-order printing.
 #include "fsgc_host.h"
 
+#include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -51,7 +53,9 @@ int write_egr(const char* path, const HostGraph& g, std::string* err)
 }
 
 // The preconditions FC:50-56 states but never checks: no self loops, no duplicates,
-// symmetric, sorted lists. Also structural sanity of nindex.
+// symmetric, sorted lists. Also structural sanity of nindex. Runs at every upload, so
+// the per-vertex passes are parallel (O(m log d) in all); the message names the
+// lowest failing vertex.
 int validate_csr(const HostGraph& g, std::string* err)
 {
   if (g.n < 1 || (int64_t)g.nindex.size() != (int64_t)g.n + 1 || (int64_t)g.nlist.size() != g.m) {
@@ -60,26 +64,44 @@ int validate_csr(const HostGraph& g, std::string* err)
   }
   if (g.m > INT32_MAX) { if (err) *err = "more than 2^31-1 directed edges"; return -22; }
   if (g.nindex[0] != 0 || g.nindex[g.n] != g.m) { if (err) *err = "nindex does not span nlist"; return -22; }
+  for (int32_t u = 0; u < g.n; u++)
+    if (g.nindex[u + 1] < g.nindex[u]) { if (err) *err = "nindex not monotone at " + std::to_string(u); return -22; }
+  const int32_t* nl = g.nlist.data();
+  const int32_t* ni = g.nindex.data();
+  // pass 1: ids in range, no self loop, strictly ascending lists
+  int32_t bad = INT32_MAX;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(min : bad)
   for (int32_t u = 0; u < g.n; u++) {
-    const int32_t b = g.nindex[u], e = g.nindex[u + 1];
-    if (e < b) { if (err) *err = "nindex not monotone at " + std::to_string(u); return -22; }
-    for (int32_t i = b; i < e; i++) {
-      const int32_t v = g.nlist[i];
-      if (v < 0 || v >= g.n) { if (err) *err = "neighbor out of range at vertex " + std::to_string(u); return -22; }
-      if (v == u) { if (err) *err = "self loop at vertex " + std::to_string(u); return -22; }
-      if (i > b && g.nlist[i - 1] >= v) { if (err) *err = "unsorted or duplicate list at vertex " + std::to_string(u); return -22; }
+    for (int32_t i = ni[u]; i < ni[u + 1]; i++) {
+      const int32_t v = nl[i];
+      if (v < 0 || v >= g.n || v == u || (i > ni[u] && nl[i - 1] >= v)) { bad = std::min(bad, u); break; }
     }
   }
-  // symmetry: every (u, v) has (v, u)
-  for (int32_t u = 0; u < g.n; u++) {
-    for (int32_t i = g.nindex[u]; i < g.nindex[u + 1]; i++) {
-      const int32_t v = g.nlist[i];
-      const int32_t* b = g.nlist.data() + g.nindex[v];
-      const int32_t* e = g.nlist.data() + g.nindex[v + 1];
-      int32_t lo = 0, hi = (int32_t)(e - b);
-      while (lo < hi) { const int32_t mid = (lo + hi) / 2; if (b[mid] < u) lo = mid + 1; else hi = mid; }
-      if (lo >= (int32_t)(e - b) || b[lo] != u) { if (err) *err = "graph is not symmetric at edge " + std::to_string(u) + "-" + std::to_string(v); return -22; }
+  if (bad != INT32_MAX) {
+    const int32_t u = bad;
+    for (int32_t i = ni[u]; i < ni[u + 1]; i++) {
+      const int32_t v = nl[i];
+      if (v < 0 || v >= g.n) { if (err) *err = "neighbor out of range at vertex " + std::to_string(u); return -22; }
+      if (v == u) { if (err) *err = "self loop at vertex " + std::to_string(u); return -22; }
     }
+    if (err) *err = "unsorted or duplicate list at vertex " + std::to_string(u);
+    return -22;
+  }
+  // pass 2 (lists are valid now): every (u, v) has (v, u)
+  bad = INT32_MAX;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(min : bad)
+  for (int32_t u = 0; u < g.n; u++) {
+    for (int32_t i = ni[u]; i < ni[u + 1]; i++) {
+      const int32_t v = nl[i];
+      const int32_t* b = nl + ni[v];
+      const int32_t* e = nl + ni[v + 1];
+      const int32_t* p = std::lower_bound(b, e, u);
+      if (p == e || *p != u) { bad = std::min(bad, u); break; }
+    }
+  }
+  if (bad != INT32_MAX) {
+    if (err) *err = "graph is not symmetric at vertex " + std::to_string(bad);
+    return -22;
   }
   return 0;
 }

@@ -24,7 +24,7 @@ struct Tune {
   int grid = 0;           // workgroups (0: 8 per CU)
   int chunk = 256;        // roots (CSR positions or vertices) per work item
   int hub_degree = 4096;  // roots of degree above this are split into single-root items
-  int lds_cap = 0;        // reserved (the LDS windows are fixed per kernel shape)
+  int lds_cap = 0;        // > 0: cap on the positions staged in LDS per window / chunk / mark array (0: the kernel's maximum)
   int sample = 1;         // > 1: run only every sample-th work item (timing runs; partial counts)
 };
 

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -250,8 +251,12 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
                                                            const TL* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
                                                            int nitems, int* __restrict__ head,
-                                                           unsigned* __restrict__ dbuf, long long dstride)
+                                                           unsigned* __restrict__ dbuf, long long dstride,
+                                                           int ecap, int achunk, int wchunk)
 {
+  // ecap <= kECap: positions per c3 window; achunk, wchunk: positions per shared phase-A
+  // chunk and per walk chunk without c3 (the kAChunk / kWalkChunk defaults, or smaller
+  // when the caller caps the LDS staging)
   __shared__ unsigned s_e[kPivWaves][kECap];
   __shared__ unsigned s_hub[kHub > 0 ? kHub : 1];
   __shared__ int s_item, s_next, s_nheavy, s_task, s_anext, s_anheavy, s_atask;
@@ -286,6 +291,15 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     if (it >= nitems) break;
     const PivotItem I = items[it];
     const int w = I.w, wb = g.beg(w), we = g.end(w), dw = we - wb;
+    if (jobs[0].smod > 1) {
+      // a root slice counts only centres u with u % smod == srem: a pivot none of whose
+      // centres is in the slice contributes nothing, so its 2-hop scatter is skipped
+      // (every thread has read s_item above, so the next fetch may overwrite it)
+      const int smod = jobs[0].smod, srem = jobs[0].srem;
+      int hit = 0;
+      for (int p = I.p0 + (int)threadIdx.x; p < I.p1 && !hit; p += kPivWaves * kWave) hit = (adj[wb + p] % smod) == srem;
+      if (!__syncthreads_or(hit)) continue;
+    }
     if (++epoch == 256) {
       for (long long i = threadIdx.x; i < dn; i += kPivWaves * kWave) D[i] = 0u;
       if (threadIdx.x == 0) bytes += 4 * dn;
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       }
       const int s = lower_bound_pos(adj, xb, xe, w + 1);
       if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1));
-      const int nch = (xe - s + kAChunk - 1) / kAChunk;
+      const int nch = (xe - s + achunk - 1) / achunk;
       if (nch > 1) {
         int slot = 0;
         if (lane == 0) slot = atomicAdd(&s_anheavy, 1);
@@ -389,8 +403,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           heavy_task(t, nh, h, k);
           const int x = adj[s_heavy[h]];
           const int xb = g.beg(x), xe = g.end(x);
-          const int s = lower_bound_pos(adj, xb, xe, w + 1) + k * kAChunk;
-          scatter2(s, min(xe, s + kAChunk));
+          const int s = lower_bound_pos(adj, xb, xe, w + 1) + k * achunk;
+          scatter2(s, min(xe, s + achunk));
         }
         __syncthreads();
       }
@@ -543,7 +557,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       if (!in_slice) continue;
       if (lane == 0) bytes += 16 + 4 * (32 - __clz(du + 1));
       const bool c3on = need_c3 && tuw > 0;
-      const int cstep = c3on ? kECap : kWalkChunk;
+      const int cstep = c3on ? ecap : wchunk;
       const int nwin = (du - (rq + 1) + cstep - 1) / cstep;
       if (nwin > 1) {
         int slot = 0;
@@ -555,9 +569,9 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         }
       }
       const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
-      // the walk past w in windows of kECap positions, each with its c3 counters in LDS
+      // the walk past w in windows of ecap positions, each with its c3 counters in LDS
       // (without c3 the whole walk is one window)
-      const int wstep = c3on ? kECap : du;
+      const int wstep = c3on ? ecap : du;
       for (int q_lo = rq + 1; q_lo < du; q_lo += wstep) window(u, ub, du, tuw, c3on, tb, te, q_lo, min(du, q_lo + wstep));
       __builtin_amdgcn_wave_barrier();
     }
@@ -583,7 +597,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         const int tuw = tsup[wb + p];
         const int rq = lower_bound_pos(adj, ub, ue, w) - ub;
         const bool c3on = need_c3 && tuw > 0;
-        const int cstep = c3on ? kECap : kWalkChunk;
+        const int cstep = c3on ? ecap : wchunk;
         const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
         const int q_lo = rq + 1 + k * cstep;
         window(u, ub, du, tuw, c3on, tb, te, q_lo, min(du, q_lo + cstep));
@@ -802,6 +816,12 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   hipStream_t stream = (hipStream_t)stream_v;
   const int nj = (int)jobs.size();
   if (nj > dev::kMaxPivotJobs) { if (err) *err = "too many jobs for one pivot launch"; return -7; }
+  // 2 * c(w, y) + 1 must fit the 24 count bits of a D entry (c(w, y) <= maxdeg); the
+  // class dispatch sends such graphs to the per-root kernel, this is the last guard
+  if (g->maxdeg >= (1 << 23)) {
+    if (err) *err = "maximum degree 2^23 or more: the pivot kernel's 24-bit 2-hop counters could overflow";
+    return -75;
+  }
   int rc = device_derived_build(g, stream_v, &lp->index_ms, err, true);
   if (rc) return rc;
   bool need_c3 = false;
@@ -873,16 +893,27 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   const uint16_t* tl16 = g->tl16;
+  // LDS staging: Tune::lds_cap > 0 caps the positions of a c3 window and of the shared
+  // phase-A and walk chunks (the launch clamps the window to its shape's array); the
+  // results do not depend on it
+  const int ecap = t.lds_cap > 0 ? t.lds_cap : 1 << 30;
+  const int achunk = t.lds_cap > 0 ? std::min(t.lds_cap, dev::kAChunk) : dev::kAChunk;
+  const int wchunk = t.lds_cap > 0 ? std::min(t.lds_cap, dev::kWalkChunk) : dev::kWalkChunk;
+  // FSGC_PIVOT_LISTS32=1 in the environment (read per launch plan) makes the kernel
+  // read the 32-bit common-neighbour lists even where the 16-bit copy exists: the path
+  // graphs of maximum degree above 65536 take, exercised on small graphs by the tests
+  const char* l32 = getenv("FSGC_PIVOT_LISTS32");
+  const bool lists32 = l32 && *l32 && *l32 != '0';
   lp->nitems = nitems;
   lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB, TL, TLP)                                                          \
   hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP, HUB, TL>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, \
                      adj, tsup, adjdeg, adjhub, tl_off, TLP, d_jobs, nj, c3, d_items + first, count, head, dbuf,   \
-                     dstride)
+                     dstride, std::min(CAP, ecap), achunk, wchunk)
 #define FSGC_PIVOT_C3(NJ, FULL, W, CAP)                                   \
   do {                                                                    \
-    if (tl16) FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, uint16_t, tl16);     \
+    if (tl16 && !lists32) FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, uint16_t, tl16); \
     else FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, int, tl);                 \
   } while (0)
 #define FSGC_PIVOT_SHAPE(NJ, FULL)                                                     \

@@ -242,9 +242,9 @@ int fsgc_server_submit(fsgc_server* s, int32_t graph, const uint8_t mo[FSGC_MO_B
   std::string err;
   const int rc = fsgc::prepare(mo, nullptr, &q->p, &err);
   if (rc) { delete q; fsgc::set_error(err); return rc; }
-  q->sched = fsgc::schedule_of(q->p, true);
   std::lock_guard<std::mutex> lk(s->mu);
   if (graph < 0 || graph >= (int32_t)s->graphs.size()) { delete q; fsgc::set_error("no such graph"); return -22; }
+  q->sched = fsgc::schedule_of(q->p, true, s->graphs[graph]->dg.maxdeg);
   if (s->stopping) { delete q; fsgc::set_error("server is stopping"); return -22; }
   q->graph = graph;
   q->ticket = s->next_ticket++;

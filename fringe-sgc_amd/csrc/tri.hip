@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
                                                              int need_lt, int tails, const TriItem* __restrict__ items,
                                                              int nitems,
                                                              int* __restrict__ head, unsigned char* __restrict__ gmark,
-                                                             long long mstride)
+                                                             long long mstride, int markcap)
 {
   __shared__ unsigned char s_mark[kTriWaves][kMarkCap];
   const int lane = lane_id();
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     if (it >= nitems) break;
     const TriItem I = items[it];
     const int x = I.x, xb = g.beg(x), dx = g.deg(x);
-    const bool lds = dx <= kMarkCap;
+    const bool lds = dx <= markcap;  // markcap <= kMarkCap
     unsigned char* __restrict__ M = lds ? s_mark[wave] : Mg;
     if (lane == 0) bytes += 16 + 8;
     for (int py = I.p0; py < I.p1; py++) {
@@ -486,12 +486,16 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   const int c3 = need_c3 ? 1 : 0, lt = need_lt ? 1 : 0, tl_arg = std::max(0, tails);
+  // Tune::lds_cap > 0 caps the degree up to which a vertex's marks sit in LDS (larger
+  // ones use the global mark array); the results do not depend on it
+  const int markcap = t.lds_cap > 0 ? std::min(t.lds_cap, dev::kMarkCap) : dev::kMarkCap;
   lp->nitems = nitems;
   lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
   hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, s, idx, adj, tsup,  \
-                     adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride)
+                     adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride, \
+                     markcap)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
     } else if (nj == 2) {

@@ -64,7 +64,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     rc = fsgc::prepare(mos + (size_t)i * FSGC_MO_BYTES, nullptr, &p, &err);
     if (rc) { fsgc::set_error(err); return rc; }
     if (p.trivial) continue;
-    by[fsgc::schedule_of(p, true)].push_back(p);
+    by[fsgc::schedule_of(p, true, g->dg.maxdeg)].push_back(p);
   }
   int nr = 0;
   for (int s = 0; s < fsgc::kNumSchedules; s++) {

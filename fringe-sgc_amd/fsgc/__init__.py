@@ -83,6 +83,7 @@ def _load():
         "fsgc_graph_copy_csr": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "fsgc_count": (C.c_int, [C.c_void_p, C.c_char_p, P(Tune), P(_Result)]),
         "fsgc_count_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P(Tune), P(_Result), P(C.c_double)]),
+        "fsgc_plan_schedule": (C.c_int, [C.c_char_p, C.c_int32, P(C.c_int32), P(C.c_int32)]),
         "fsgc_preprocess": (C.c_int, [C.c_int, C.c_int, P(C.c_int), C.c_char_p, C.c_char_p, C.c_size_t,
                                       C.c_char_p, C.c_size_t]),
         "fsgc_preprocess_pattern": (C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),
@@ -119,12 +120,16 @@ lib = _load()
 
 
 class FsgcError(RuntimeError):
-    pass
+    """A negative return code of the C ABI (`code`) with its message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
 
 
 def _check(rc: int):
     if rc != 0:
-        raise FsgcError(f"{lib.fsgc_strerror(rc).decode()} ({rc}): {lib.fsgc_last_error().decode()}")
+        raise FsgcError(rc, f"{lib.fsgc_strerror(rc).decode()} ({rc}): {lib.fsgc_last_error().decode()}")
 
 
 @dataclass
@@ -164,6 +169,14 @@ def motif_class(mo: bytes) -> str:
     if nodes == 3:
         return "wedge" if bin(mo[9]).count("1") == 1 else "triangle"
     return "general"
+
+
+def plan_schedule(mo: bytes, maxdeg: int) -> str:
+    """Name of the kernel schedule fsgc_count uses for `mo` on a graph of maximum
+    degree `maxdeg` ("host" for the vertex and edge counts). Host only."""
+    s, c = C.c_int32(), C.c_int32()
+    _check(lib.fsgc_plan_schedule(mo, maxdeg, C.byref(s), C.byref(c)))
+    return "host" if s.value < 0 else SCHEDULE_NAMES[s.value]
 
 
 def init(device: int = 0):

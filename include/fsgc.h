@@ -72,8 +72,10 @@ typedef struct fsgc_tune {
   int32_t grid;        /* persistent workgroups; 0 = tuned or default (4-8 per CU) */
   int32_t chunk;       /* roots per work item; 0 = tuned or default */
   int32_t hub_degree;  /* roots above this degree get items of their own; 0 = tuned or default */
-  int32_t lds_cap;     /* reserved, ignored: each kernel shape fixes its LDS window (pivot: 4096 positions
-                          per wave for 256/512 threads, 2048 for 1024) */
+  int32_t lds_cap;     /* > 0: caps the list positions staged in LDS: the pivot kernel's c3 windows (at most
+                          4096 positions per wave for 256/512 threads, 2048 for 1024) and its shared
+                          phase-A and walk chunks (2048), and the degree up to which the rank kernel keeps
+                          a vertex's marks in LDS (8192; larger ones use global memory); 0 = those maxima */
   int32_t slice_mod;   /* > 1: count only roots u with u % slice_mod == slice_rem */
   int32_t slice_rem;
   int32_t schedule;    /* 0: fastest schedule per class; 1: per-root schedule (the reference's) for every class */
@@ -83,7 +85,10 @@ typedef struct fsgc_tune {
 int fsgc_init(int device);
 
 /* Graph residency. The graph must be undirected with sorted, duplicate-free lists and
-   no self loops (fringeCount.cu:50-56); fsgc_graph_validate checks that. */
+   no self loops (fringeCount.cu:50-56, which states this but never checks it).
+   fsgc_graph_from_csr, fsgc_graph_load_egr and fsgc_graph_generate check it on the
+   host before anything reaches the GPU and return -22 (message in fsgc_last_error)
+   for a malformed graph; fsgc_graph_validate re-checks a resident graph. */
 int fsgc_graph_load_egr(const char* path, fsgc_graph** out);
 int fsgc_graph_from_csr(int32_t nodes, int32_t edges, const int32_t* nindex, const int32_t* nlist, fsgc_graph** out);
 int fsgc_graph_validate(const fsgc_graph* g);
@@ -111,6 +116,14 @@ int fsgc_graph_copy_csr(const fsgc_graph* g, int32_t* nindex, int32_t* nlist);
 /* Counts one motif (one persistent launch on the data graph, plus one on the motif
    graph for the automorphism sum, as the reference does). tune may be NULL. */
 int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* tune, fsgc_result* out);
+
+/* The kernel schedule fsgc_count uses for a motif on a graph of maximum degree maxdeg:
+   0 per-root, 1 pivot, 2 rank, 3 edge pass, -1 none (vertex or edge count, computed on
+   the host). Wedge cores take the per-root schedule when maxdeg >= 2^23, where the
+   pivot kernel's 24-bit 2-hop counters could overflow. Host only: no GPU is touched.
+   No reference counterpart (the reference's dispatch sits inside occurrences,
+   fringeCount.cu:2199-2262, and has one kernel per class). motif_class may be NULL. */
+int fsgc_plan_schedule(const uint8_t mo[FSGC_MO_BYTES], int32_t maxdeg, int32_t* schedule, int32_t* motif_class);
 
 /* Counts k motifs in one persistent launch over one graph; out[i] is identical to
    fsgc_count of mos + i*139 alone. batch_ms receives the device time of the launch. */

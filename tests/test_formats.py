@@ -42,10 +42,16 @@ def test_generators_are_deterministic_and_valid():
 
 
 def test_er_config_shape():
+    """G(4096, 32768) from seed 1: 32,768 distinct undirected pairs, no self loops, and
+    a different seed gives a different graph of the same size."""
     ni, nl = fsgc.generate_csr("er:4096:32768:1")
     assert len(ni) == 4097 and len(nl) == 65536
-    h = hashlib.sha256(ni.tobytes() + nl.tobytes()).hexdigest()
-    assert h == hashlib.sha256(ni.tobytes() + nl.tobytes()).hexdigest()
+    src = np.repeat(np.arange(4096), np.diff(ni))
+    pairs = {(int(a), int(b)) for a, b in zip(src, nl) if a < b}
+    assert len(pairs) == 32768
+    ni2, nl2 = fsgc.generate_csr("er:4096:32768:2")
+    assert len(nl2) == 65536
+    assert hashlib.sha256(ni.tobytes() + nl.tobytes()).digest() != hashlib.sha256(ni2.tobytes() + nl2.tobytes()).digest()
 
 
 def test_chung_lu_degree_profile():
@@ -105,19 +111,22 @@ def test_reproduce_library_matches_reference_families():
                      "fig3wedge": 11}
 
 
-def test_config_graphs_match_committed_sha256():
-    """The generators still produce the committed benchmark graphs (C1 and C2 here; the
-    larger ones are in the same file and are regenerated by make_graph_sha.py)."""
+@pytest.mark.parametrize("spec", ["er:4096:32768:1", "rmat:20:16:0.57:0.19:0.19:7", "rmat:22:16:0.57:0.19:0.19:11",
+                                  "chunglu:4000000:2.1:20:100000:19", "rmat:23:16:0.57:0.19:0.19:23",
+                                  "rmat:21:32:0.57:0.19:0.19:29"])
+def test_config_graphs_match_committed_sha256(spec):
+    """The generators still produce the committed benchmark graphs: the ER, R-MAT and
+    Chung-Lu graphs of all five configurations (graph_sha256.json, written by
+    make_graph_sha.py)."""
     import hashlib
     import json
     ref = json.load(open(os.path.join(ROOT, "tests", "golden", "graph_sha256.json")))
-    for spec in ("er:4096:32768:1", "rmat:20:16:0.57:0.19:0.19:7"):
-        ni, nl = fsgc.generate_csr(spec)
-        h = hashlib.sha256()
-        h.update(ni.astype("<i4").tobytes())
-        h.update(nl.astype("<i4").tobytes())
-        assert h.hexdigest() == ref[spec]["sha256"], spec
-        assert len(nl) == ref[spec]["directed_edges"]
+    ni, nl = fsgc.generate_csr(spec)
+    h = hashlib.sha256()
+    h.update(ni.astype("<i4").tobytes())
+    h.update(nl.astype("<i4").tobytes())
+    assert h.hexdigest() == ref[spec]["sha256"], spec
+    assert len(nl) == ref[spec]["directed_edges"]
 
 
 def test_relabel_order_is_the_reference_key_order():

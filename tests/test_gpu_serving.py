@@ -105,3 +105,63 @@ def test_server_rejects_query_over_cap(gpu, graphs):
         srv.wait(t)
     assert srv.stats()["rejected"] == 1
     srv.close()
+
+
+def test_served_c5_stream_vs_oracle(gpu):
+    """The served stream of the two-graph configuration, scaled down to sizes the oracle
+    covers: the same generators, seeds and edge factors (R-MAT ef16 seed 23 and ef32
+    seed 29), motifs of every class from the same generator (seed 5), Poisson arrivals
+    from seed 5 submitted at their arrival times while earlier queries run. Every
+    served result is compared with the oracle. Cores of 4-5 vertices go to the scale-4
+    pair (the oracle's general-core fringe enumeration is exponential), the others to
+    the scale-10/9 pair."""
+    import threading
+    import time
+
+    import numpy as np
+
+    import oracle
+
+    pairs = {"small": ["rmat:4:16:0.57:0.19:0.19:23", "rmat:4:32:0.57:0.19:0.19:29"],
+             "large": ["rmat:10:16:0.57:0.19:0.19:23", "rmat:9:32:0.57:0.19:0.19:29"]}
+    csr = {s: gpu.generate_csr(s) for p in pairs.values() for s in p}
+    rng = np.random.default_rng(5)
+    mos = gpu.random_motifs(5, 64)
+    classes = {gpu.motif_class(m) for m in mos}
+    assert classes == {"vertex", "edge", "wedge", "triangle", "general"}
+    queries = []
+    for mo in mos:
+        side = "small" if gpu.motif_class(mo) == "general" else "large"
+        queries.append((pairs[side][int(rng.integers(2))], mo))
+    gaps = rng.exponential(1.0 / 200.0, size=len(queries))  # 200 queries/s offered
+    srv = gpu.Server(slots=2)
+    graphs = {s: gpu.Graph.from_csr(*csr[s]) for s in csr}
+    ids = {s: srv.add_graph(g) for s, g in graphs.items()}
+    tickets = [None] * len(queries)
+
+    def submitter():
+        start = time.perf_counter()
+        at = 0.0
+        for i, (s, mo) in enumerate(queries):
+            at += gaps[i]
+            d = start + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(ids[s], mo)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    got = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    print("served", st, flush=True)
+    assert st["completed"] == len(queries) and st["rejected"] == 0
+    assert max(q.batch_size for q in got) > 1
+    from test_gpu_parity import same
+    for i, ((s, mo), q) in enumerate(zip(queries, got)):
+        print("query", i, gpu.motif_class(mo), s, flush=True)
+        assert q.status == 0
+        same(q.result, oracle.count(mo, *csr[s]), f"query {i} ({gpu.motif_class(mo)}) on {s}")
+    for g in graphs.values():
+        g.free()
