@@ -78,90 +78,170 @@ __device__ __forceinline__ int lower_bound_pos(const int* __restrict__ a, int l,
 
 // ---------------------------------------------------------------- support pass
 
-// Per directed edge (u, x): |N(u) & N(x)| (count mode) or the ascending positions in
-// N(u) of the common neighbours (list mode). Light edges (shorter list <= 32) are done
-// by one lane; heavier ones cooperatively by the wave.
+// Per-edge common neighbours, built once per graph: the supports t(u,x) = |N(u) & N(x)|
+// (count pass) and the lists T(u,x) of the positions in N(u) of those common
+// neighbours, ascending (list pass, after a scan of the supports gives the offsets).
+//
+// Each undirected edge {u, x} is intersected once, from its endpoint of higher rank
+// (degree, then id), u: a wave stages N(u) in LDS (every stride-th id of it when it is
+// longer than kComStage, the search then ending in one block of the global list) and
+// streams the lists N(x) of u's lower-ranked neighbours x, packed one after another
+// across its 64 lanes, so short lists share a wave and long ones are read coalesced.
+// Each element y at position j of N(x) is searched in the staged N(u); a hit at
+// position i is entry r of both T(x,u) (= j) and T(u,x) (= i), r being the number of
+// earlier hits of the pair: both lists ascend with y. Cost: the shorter list of every
+// edge streamed once, with log2(d_u) LDS probes per element.
+constexpr int kComStage = 2048;  // ids of N(u) staged per wave (8 KB)
+constexpr int kComBlock = 256;
+constexpr int kComWaves = kComBlock / kWave;
+
+struct ComItem {
+  int32_t u;
+  int32_t p0;  // positions [p0, p1) of N(u) whose lower-ranked neighbours this item takes
+  int32_t p1;
+  int32_t pad;
+};
+
 template <bool kList>
-__global__ __launch_bounds__(256) void k_common(const int* __restrict__ idx, const int* __restrict__ adj, int n,
-                                                int* __restrict__ tsup, const long long* __restrict__ tl_off,
-                                                int* __restrict__ tl, int* __restrict__ head)
+__global__ __launch_bounds__(kComBlock) void k_common(const int* __restrict__ idx, const int* __restrict__ adj,
+                                                      const int* __restrict__ adjdeg,
+                                                      const ComItem* __restrict__ items, int nitems,
+                                                      int* __restrict__ head, int* __restrict__ tsup,
+                                                      const long long* __restrict__ tl_off, int* __restrict__ tl)
 {
+  __shared__ int s_stage[kComWaves][kComStage];
+  // per wave, one entry per lane's x: offset of its list in the packed stream (entry 64
+  // = total), list start in adj, hits so far, and (list pass) the output bases
+  __shared__ int s_off[kComWaves][kWave + 1];
+  __shared__ int s_xb[kComWaves][kWave];
+  __shared__ int s_cnt[kComWaves][kWave];
+  __shared__ long long s_bux[kComWaves][kList ? kWave : 1];
+  __shared__ long long s_bxu[kComWaves][kList ? kWave : 1];
+  __shared__ int s_lim[kComWaves][kList ? kWave : 1];  // the pair's support from the count pass
   const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  int* __restrict__ S = s_stage[wave];
+  int* __restrict__ off = s_off[wave];
+  int* __restrict__ sxb = s_xb[wave];
+  int* __restrict__ cnt = s_cnt[wave];
   for (;;) {
-    int u = 0;
-    if (lane == 0) u = atomicAdd(head, 1);
-    u = bcast(u, 0);
-    if (u >= n) break;
-    const int ub = idx[u], ue = idx[u + 1], du = ue - ub;
-    for (int e0 = ub; e0 < ue; e0 += kWave) {
-      const int e = e0 + lane;
-      bool heavy = false;
-      int x = 0, xb = 0, xe = 0;
-      if (e < ue) {
-        x = adj[e];
+    int it = 0;
+    if (lane == 0) it = atomicAdd(head, 1);
+    it = bcast(it, 0);
+    if (it >= nitems) break;
+    const ComItem I = items[it];
+    const int u = I.u, ub = idx[u], du = idx[u + 1] - ub;
+    const int stride = (du + kComStage - 1) / kComStage;  // 1 when N(u) fits
+    const int ns = (du + stride - 1) / stride;
+    __builtin_amdgcn_wave_barrier();  // the previous item's readers of S are done
+    for (int k = lane; k < ns; k += kWave) S[k] = adj[ub + k * stride];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // position of y in N(u), or -1
+    auto find = [&](int y) -> int {
+      if (S[0] > y) return -1;
+      int lo = 0, hi = ns;  // S[lo] <= y < S[hi] (S[ns] = +inf)
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (S[mid] <= y) lo = mid; else hi = mid;
+      }
+      if (stride == 1) return S[lo] == y ? lo : -1;
+      int l = ub + lo * stride + 1, r = min(ub + (lo + 1) * stride, ub + du);
+      if (S[lo] == y) return lo * stride;
+      const int r0 = r;
+      while (l < r) {
+        const int mid = (l + r) >> 1;
+        if (adj[mid] < y) l = mid + 1; else r = mid;
+      }
+      return (l < r0 && adj[l] == y) ? l - ub : -1;
+    };
+    for (int q0 = I.p0; q0 < I.p1; q0 += kWave) {
+      const int p = q0 + lane;
+      int x = 0, dx = 0, xb = 0, exu = 0;
+      bool take = false;
+      if (p < I.p1) {
+        x = adj[ub + p];
+        dx = adjdeg[ub + p];
+        take = dx < du || (dx == du && x < u);  // u ranks higher: the edge is intersected here
+      }
+      if (take) {
         xb = idx[x];
-        xe = idx[x + 1];
-        const int dx = xe - xb;
-        const int shorter = min(du, dx);
-        if (shorter <= 32) {
-          // serial: walk the shorter list, search the longer one
-          int c = 0;
-          long long out = kList ? tl_off[e] : 0;
-          if (du <= dx) {
-            for (int i = ub; i < ue; i++) {
-              if (contains(adj, xb, xe, adj[i])) {
-                if (kList) tl[out++] = i - ub;
-                c++;
-              }
-            }
-          } else {
-            for (int i = xb; i < xe; i++) {
-              const int y = adj[i];
-              const int p = lower_bound_pos(adj, ub, ue, y);
-              if (p < ue && adj[p] == y) {
-                if (kList) tl[out++] = p - ub;
-                c++;
-              }
-            }
-          }
-          if (!kList) tsup[e] = c;
-        } else {
-          heavy = true;
+        int l = xb, r = xb + dx;  // position of u in N(x): the reverse edge (x, u)
+        while (l < r) {
+          const int mid = (l + r) >> 1;
+          if (adj[mid] < u) l = mid + 1; else r = mid;
         }
+        exu = l;
       }
-      ull hm = __ballot(heavy);
-      while (hm) {
-        const int who = __builtin_ctzll(hm);
-        hm &= hm - 1;
-        const int X = bcast(x, who), XB = bcast(xb, who), XE = bcast(xe, who), E = bcast(e, who);
-        const int dx = XE - XB;
-        const bool walk_u = du <= dx;
-        const int sb = walk_u ? ub : XB, se = walk_u ? ue : XE;
-        int c = 0;
-        long long out = kList ? tl_off[E] : 0;
-        for (int i0 = sb; i0 < se; i0 += kWave) {
-          const int i = i0 + lane;
-          bool hit = false;
-          int pos = 0;
-          if (i < se) {
-            const int y = adj[i];
-            if (walk_u) {
-              hit = contains(adj, XB, XE, y);
-              pos = i - ub;
-            } else {
-              const int p = lower_bound_pos(adj, ub, ue, y);
-              hit = p < ue && adj[p] == y;
-              pos = p - ub;
-            }
-          }
-          const ull b = __ballot(hit);
-          if (kList && hit) tl[out + __popcll(b & ((1ull << lane) - 1))] = pos;
-          out += __popcll(b);
-          c += __popcll(b);
+      const int len = take ? dx : 0;
+      int incl = len;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+      }
+      const int tot = __shfl(incl, kWave - 1, kWave);
+      off[lane] = incl - len;
+      if (lane == kWave - 1) off[kWave] = tot;
+      sxb[lane] = xb;
+      cnt[lane] = 0;
+      if constexpr (kList) {
+        s_bux[wave][lane] = take ? tl_off[ub + p] : 0;
+        s_bxu[wave][lane] = take ? tl_off[exu] : 0;
+        s_lim[wave][lane] = take ? tsup[ub + p] : 0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // the packed stream: element g belongs to the last lane k with off[k] <= g (an x
+      // with an empty list shares its offset with the next one, which wins)
+      int k = 0;
+      {
+        int lo = 0, hi = kWave;  // off[lo] <= g < off[hi]
+        const int g0 = min(lane, max(tot - 1, 0));
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (off[mid] <= g0) lo = mid; else hi = mid;
         }
-        if (!kList && lane == 0) tsup[E] = c;
-        (void)X;
+        k = lo;
       }
+      for (int c = 0; c < tot; c += kWave) {
+        const int gi = c + lane;
+        const bool act = gi < tot;
+        int i = -1, j = 0;
+        if (act) {
+          while (off[k + 1] <= gi) k++;
+          j = gi - off[k];
+          i = find(adj[sxb[k] + j]);
+        }
+        const bool hit = i >= 0;
+        const unsigned long long hm = __ballot(hit);
+        // lanes of segment k in this chunk: [first, last]
+        const int first = act ? max(off[k] - c, 0) : 0;
+        const int last = act ? min(off[k + 1] - c, kWave) - 1 : 0;
+        const unsigned long long below = (lane == 0 ? 0ull : (~0ull >> (kWave - lane))) & ~(first == 0 ? 0ull : (~0ull >> (kWave - first)));
+        if (kList && hit) {
+          const int r = cnt[k] + __popcll(hm & below);
+          // the count pass found the same hits, so r < the support; the test keeps every
+          // store inside the pair's two lists whatever happens
+          if (r < s_lim[wave][k]) {
+            tl[s_bux[wave][k] + r] = i;
+            tl[s_bxu[wave][k] + r] = j;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (act && lane == last) cnt[k] += __popcll(hm & (below | (1ull << lane)));
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (!kList && take) {
+        const int t = cnt[lane];
+        tsup[ub + p] = t;
+        tsup[exu] = t;
+      }
+      __builtin_amdgcn_wave_barrier();  // cnt, off and sxb are reused by the next batch
     }
   }
 }
@@ -708,7 +788,36 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   int dev = 0, cus = 256;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::max(1, std::min(cus * 8, (n + 3) / 4));
+  // work items of the intersection passes: each vertex u with its positions cut into
+  // ranges (256 positions above 1024), highest degree first so the longest items start
+  // early; a counting sort over the degrees
+  dev::ComItem* d_citems = nullptr;
+  int ncitems = 0;
+  auto common_items = [&]() -> int {
+    if (d_citems) return 0;
+    const int md = g->maxdeg;
+    std::vector<long long> start((size_t)md + 2, 0);
+    for (int v = 0; v < n; v++) {
+      const int d = g->h_idx[v + 1] - g->h_idx[v];
+      if (d > 0) start[(size_t)(md - d) + 1] += d <= 1024 ? 1 : (d + 255) / 256;
+    }
+    for (size_t i = 1; i < start.size(); i++) start[i] += start[i - 1];
+    std::vector<dev::ComItem> items((size_t)start[(size_t)md + 1]);
+    for (int v = 0; v < n; v++) {
+      const int d = g->h_idx[v + 1] - g->h_idx[v];
+      if (d == 0) continue;
+      long long& at = start[(size_t)(md - d)];
+      const int step = d <= 1024 ? d : 256;
+      for (int p = 0; p < d; p += step) items[(size_t)at++] = dev::ComItem{v, p, std::min(d, p + step), 0};
+    }
+    ncitems = (int)items.size();
+    if (sc.get(&d_citems, sizeof(dev::ComItem) * std::max<size_t>(1, items.size())) != hipSuccess) return -5;
+    if (!items.empty() && hipMemcpyAsync(d_citems, items.data(), sizeof(dev::ComItem) * items.size(),
+                                         hipMemcpyHostToDevice, stream) != hipSuccess)
+      return -5;
+    return hipStreamSynchronize(stream) == hipSuccess ? 0 : -5;  // the host items go out of scope
+  };
+  const int cgrid = cus * 8;
   if (!g->support_ready) {
     HIP_TRY(hipMalloc((void**)&g->tsup, sizeof(int) * std::max<long long>(1, m)));
     HIP_TRY(hipMalloc((void**)&g->piv_cost, sizeof(long long) * std::max(1, n)));
@@ -733,8 +842,9 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(stream));  // the host slot array goes out of scope
     }
-    hipLaunchKernelGGL(dev::k_common<false>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
-                       (const long long*)nullptr, (int*)nullptr, d_head);
+    if (common_items()) { if (err) *err = "upload of the intersection work items failed"; return -5; }
+    hipLaunchKernelGGL(dev::k_common<false>, dim3(cgrid), dim3(dev::kComBlock), 0, stream, g->idx, g->adj,
+                       g->adjdeg, d_citems, ncitems, d_head, g->tsup, (const long long*)nullptr, (int*)nullptr);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
     HIP_TRY(hipGetLastError());
@@ -760,8 +870,9 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     g->tl_total = total;
     HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
     HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
-    hipLaunchKernelGGL(dev::k_common<true>, dim3(grid), dim3(256), 0, stream, g->idx, g->adj, n, g->tsup,
-                       (const long long*)g->tl_off, g->tl, d_head + 1);
+    if (common_items()) { if (err) *err = "upload of the intersection work items failed"; return -5; }
+    hipLaunchKernelGGL(dev::k_common<true>, dim3(cgrid), dim3(dev::kComBlock), 0, stream, g->idx, g->adj,
+                       g->adjdeg, d_citems, ncitems, d_head + 1, g->tsup, (const long long*)g->tl_off, g->tl);
     HIP_TRY(hipGetLastError());
     g->derived_bytes += sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total;
     // 16-bit copy of the lists for the pivot kernel: a position in N(u) is below
