@@ -202,6 +202,37 @@ int fsgc_count_batch(fsgc_graph* g, const uint8_t* mos, int k, const fsgc_tune* 
   return 0;
 }
 
+int fsgc_count_noalloc(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], fsgc_result* out)
+{
+  if (!g || !mo || !out) return fail(-22, "null argument");
+  const double t0 = now_ms();
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (hipSetDevice(fsgc::current_device()) != hipSuccess) return fail(-5, "cannot select the GPU");
+  fsgc::Prepared p;
+  std::string err;
+  rc = fsgc::prepare(mo, nullptr, &p, &err);
+  if (rc) return fail(rc, err);
+  fsgc::JobResult r;
+  double ms = 0;
+  if (!p.trivial) {
+    if (fsgc::schedule_of(p, true, g->dg.maxdeg) != fsgc::kEdgePass)
+      return fail(-95, "no-allocation count: only vertex motifs and edge motifs of the closed-form kind "
+                       "(counted by the edge pass) are supported");
+    if (!g->dg.support_ready)
+      return fail(-11, "no-allocation count: the graph's per-edge supports are not built yet (any fsgc_count "
+                       "of an edge, wedge or triangle motif on the graph builds them)");
+    rc = fsgc::edgepass_count_noalloc(&g->dg, p.job, &r, &ms, &err);
+    if (rc) return fail(rc, err);
+  }
+  std::memset(out, 0, sizeof(fsgc_result));
+  fsgc::finish(p, r, g->dg.n, g->dg.m, out);
+  out->kernel_ms = ms;
+  out->wall_ms = now_ms() - t0;
+  out->alg_bytes = (double)r.bytes;
+  return 0;
+}
+
 int fsgc_plan_schedule(const uint8_t mo[FSGC_MO_BYTES], int32_t maxdeg, int32_t* schedule, int32_t* motif_class)
 {
   if (!mo || !schedule) return fail(-22, "null argument");

@@ -133,6 +133,10 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
                         std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                          std::vector<JobResult>* out, double* kernel_ms, std::string* err);
+// One edge-pass job (closed-form edge core or stars) with all its device scratch in a
+// fixed array (no allocation); the graph's supports must be built. kernel_ms: host
+// time of the device part.
+int edgepass_count_noalloc(const DeviceGraph* g, JobDesc job, JobResult* out, double* kernel_ms, std::string* err);
 
 // kernels.hip
 int device_init(int device, std::string* err);

@@ -15,6 +15,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "devmath.h"
@@ -422,6 +425,97 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
   LaunchPlan lp;
   int rc = plan_edgepass(jobs, g, t, stream, &lp, err);
   return rc ? rc : launch_run(lp, stream, out, kernel_ms, err);
+}
+
+// ---------------------------------------------------------------- no-allocation count
+//
+// One edge pass for one motif with every piece of device scratch it needs (work-queue
+// head, accumulator, job descriptor, work items) in a fixed array of this file, so the
+// call allocates nothing. The array's device address comes from hipGetSymbolAddress,
+// once per device; kernels and copies only ever see that address. Calls on one device
+// share the array: a mutex serialises them within the process.
+constexpr size_t kNoAllocScratch = 4u << 20;  // bytes
+constexpr size_t kNoAllocJobOff = 512;         // [0, 512): queue head (0) and the accumulator (256)
+constexpr size_t kNoAllocItemsOff = kNoAllocJobOff + (sizeof(JobDesc) + 255) / 256 * 256;
+constexpr long long kNoAllocStep = 8192;       // CSR positions per work item, as the allocating edge pass
+constexpr int kNoAllocMaxItems = (int)((kNoAllocScratch - kNoAllocItemsOff) / sizeof(WorkItem));
+constexpr int kNoAllocMaxDevices = 64;
+
+namespace dev {
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+// clears the head and the accumulator, and writes the work items of the edge pass:
+// item i covers CSR positions [i * step, min(m, (i + 1) * step)) with the owners of
+// its first and last position (the last vertex u with idx[u] <= position)
+__global__ void k_noalloc_setup(const int* __restrict__ idx, int n, long long m, long long step,
+                                unsigned char* __restrict__ base, int nitems)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (int)(kNoAllocJobOff / 8)) reinterpret_cast<ull*>(base)[t] = 0ull;
+  if (t >= nitems) return;
+  WorkItem* items = reinterpret_cast<WorkItem*>(base + kNoAllocItemsOff);
+  const long long b = (long long)t * step, e = min(m, b + step);
+  auto owner = [&](long long pos) {
+    int lo = 0, hi = n;  // idx[lo] <= pos < idx[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((long long)idx[mid] <= pos) lo = mid; else hi = mid;
+    }
+    return lo;
+  };
+  items[t] = WorkItem{0, owner(b), owner(e - 1), 0, b, e};
+}
+
+}  // namespace dev
+
+int edgepass_count_noalloc(const DeviceGraph* g, JobDesc job, JobResult* out, double* kernel_ms, std::string* err)
+{
+  static std::mutex mu;
+  static unsigned char* ptr[kNoAllocMaxDevices] = {};
+  std::lock_guard<std::mutex> lk(mu);
+  int dev = 0, cus = 256;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) { if (err) *err = "device index out of range"; return -19; }
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(dev::scratch)));
+    ptr[dev] = (unsigned char*)p;
+  }
+  unsigned char* base = ptr[dev];
+  const long long m = g->m;
+  const long long nitems = (m + kNoAllocStep - 1) / kNoAllocStep;
+  if (nitems > kNoAllocMaxItems) {
+    if (err) *err = "graph too large for the no-allocation scratch: " + std::to_string(m) + " directed edges, at most " +
+                    std::to_string((long long)kNoAllocMaxItems * kNoAllocStep);
+    return -27;
+  }
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  job.idx = g->idx; job.adj = g->adj; job.n = g->n; job.m = (int32_t)m;
+  job.out = reinterpret_cast<Accum*>(base + 256);
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipMemcpy(base + kNoAllocJobOff, &job, sizeof(JobDesc), hipMemcpyHostToDevice));
+  const int setup = std::max<int>((int)nitems, (int)(kNoAllocJobOff / 8));
+  hipLaunchKernelGGL(dev::k_noalloc_setup, dim3((setup + 255) / 256), dim3(256), 0, 0, (const int*)g->idx, g->n, m,
+                     kNoAllocStep, base, (int)nitems);
+  HIP_TRY(hipGetLastError());
+  if (nitems > 0) {
+    const int grid = std::max(1, std::min<int>(cus * 8, (int)((nitems + 3) / 4)));
+    hipLaunchKernelGGL((dev::k_edge_pass<1>), dim3(grid), dim3(256), 0, 0, (const int*)g->idx, (const int*)g->adj,
+                       (const int*)g->tsup, (const int*)g->adjdeg, reinterpret_cast<const JobDesc*>(base + kNoAllocJobOff),
+                       1, reinterpret_cast<const WorkItem*>(base + kNoAllocItemsOff), (int)nitems,
+                       reinterpret_cast<int*>(base));
+    HIP_TRY(hipGetLastError());
+  }
+  Accum a;
+  HIP_TRY(hipMemcpy(&a, base + 256, sizeof(Accum), hipMemcpyDeviceToHost));  // waits for the kernels
+  if (kernel_ms) *kernel_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  out->sum = ((unsigned __int128)a.hi << 64) | a.lo;
+  out->matches = a.matches;
+  out->flags = a.flags;
+  out->bytes = a.bytes;
+  out->index_ms = 0;
+  return 0;
 }
 
 // Prepares one rank-schedule launch of up to kMaxTriJobs triangle-based jobs.

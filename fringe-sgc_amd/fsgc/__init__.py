@@ -84,6 +84,7 @@ def _load():
         "fsgc_count": (C.c_int, [C.c_void_p, C.c_char_p, P(Tune), P(_Result)]),
         "fsgc_count_batch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, P(Tune), P(_Result), P(C.c_double)]),
         "fsgc_plan_schedule": (C.c_int, [C.c_char_p, C.c_int32, P(C.c_int32), P(C.c_int32)]),
+        "fsgc_count_noalloc": (C.c_int, [C.c_void_p, C.c_char_p, P(_Result)]),
         "fsgc_preprocess": (C.c_int, [C.c_int, C.c_int, P(C.c_int), C.c_char_p, C.c_char_p, C.c_size_t,
                                       C.c_char_p, C.c_size_t]),
         "fsgc_preprocess_pattern": (C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]),
@@ -251,6 +252,13 @@ class Graph:
         assert len(mo) == MO_BYTES
         r = _Result()
         _check(lib.fsgc_count(self._h, mo, C.byref(tune) if tune else None, C.byref(r)))
+        return _result(r)
+
+    def count_noalloc(self, mo: bytes) -> Result:
+        """fsgc_count_noalloc: the same result as count() for vertex and closed-form
+        edge motifs, with no allocation (the graph's supports must be built)."""
+        r = _Result()
+        _check(lib.fsgc_count_noalloc(self._h, mo, C.byref(r)))
         return _result(r)
 
     def count_batch(self, mos: list[bytes], tune: Tune | None = None):

@@ -117,6 +117,20 @@ int fsgc_graph_copy_csr(const fsgc_graph* g, int32_t* nindex, int32_t* nlist);
    graph for the automorphism sum, as the reference does). tune may be NULL. */
 int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* tune, fsgc_result* out);
 
+/* fsgc_count for callers that cannot allocate (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc). No reference counterpart. It calls
+   no hipMalloc, hipFree or other allocating routine: its device scratch (work-queue
+   head, accumulator, job descriptor, work items) is a fixed 4 MiB array of the library,
+   one copy per GPU, held until the process ends. Supported: vertex motifs and edge
+   motifs of the closed-form kind (the ones fsgc_plan_schedule maps to the edge pass,
+   or to the host); others return -95. The graph's per-edge supports must already be
+   built (any fsgc_count of an edge, wedge or triangle motif on g builds them), else
+   -11. Graphs of more than 130,960 * 8192 = 1,072,824,320 directed edges (the work items
+   of 8192 edges each then fill the array) return -27.
+   Calls on one GPU share the array; the library serialises them, so concurrent calls
+   wait for each other. out equals fsgc_count's result; kernel_ms is host-timed. */
+int fsgc_count_noalloc(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], fsgc_result* out);
+
 /* The kernel schedule fsgc_count uses for a motif on a graph of maximum degree maxdeg:
    0 per-root, 1 pivot, 2 rank, 3 edge pass, -1 none (vertex or edge count, computed on
    the host). Wedge cores take the per-root schedule when maxdeg >= 2^23, where the
