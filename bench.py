@@ -274,6 +274,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune the launch shape on the workload first")
     ap.add_argument("--tune-sample", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    ap.add_argument("--alg-model", action="store_true",
+                    help="recompute the pivot kernel's algorithmic bytes from unit counts on the host (oracle model) "
+                         "and check them against the kernel's (c2 workload)")
     ap.add_argument("--gpu-check-budget", type=float, default=60.0,
                     help="seconds of GPU slice counts checked against the CPU baseline's slices")
     ap.add_argument("--batch", default=None, help="graph spec: compare plain vs batched over the random motifs")
@@ -413,6 +416,20 @@ def main():
         }
         if not parity:
             print("# CPU/GPU slice mismatch!", file=sys.stderr)
+        if args.alg_model and W["kernel"] == "k_wedge_pivot":
+            # the unit counts of the timed launch, from the CSR it ran on, and the
+            # per-unit sizes of the kernel: they must give the kernel's own figure
+            t1 = time.perf_counter()
+            cni, cnl = gc.csr()
+            units = oracle.pivot_units(cni, cnl, ecap=2048, hub_slots=0, c3=True, threads=cores)
+            model = oracle.pivot_bytes(units, 2 if maxdeg <= 65536 else 4, hub_launch=False)
+            roofline["alg_model"] = {"units": units, "bytes_per_unit": "20 paths_d + 4 paths_hub + 16 marks_d + 4 marks_hub + "
+                                     "12 xvisits + 4 xvisit_bits + 16 centres + 4 centre_bits + (16+s) c3_x + "
+                                     "(8+s) c3_search_bits + s list_elems + 16 walk_d + 12 walk_hub, s = list element "
+                                     "bytes", "list_elem_bytes": 2 if maxdeg <= 65536 else 4, "bytes": model,
+                                     "equals_kernel": int(r.alg_bytes) == model,
+                                     "seconds": round(time.perf_counter() - t1, 2)}
+            print(f"# alg-bytes model: {model} (kernel {int(r.alg_bytes)}), units {units}", flush=True)
 
     # the other two parts of the metric, on small graphs so that the run stays within
     # minutes (the full-size runs: bench.py --batch / --serve, results under profiles/)

@@ -15,7 +15,8 @@
 //     with v in T(u,x): a scatter over the precomputed lists into LDS counters E
 //     indexed by position in N(u), one window of kECap positions at a time (hub
 //     centres take several windows; each x's list is cut to the window by two
-//     binary searches);
+//     binary searches); the cut lists of a wave's 64 x's are streamed as one
+//     sequence, so short and long lists alike keep eight loads per lane in flight;
 //   * a wave then walks the window and evaluates every wedge job per lane;
 //   * within a workgroup the waves share the work of one pivot through LDS task
 //     lists: neighbours x with long lists (2-hop scatter) and centres u with several
@@ -50,7 +51,7 @@ constexpr int kMaxPivotJobs = 8;
 // in the global per-workgroup array D. (Launches with c3 keep their 128 KB of windows:
 // 8184 hubs in the remaining LDS measured 2% slower than none, the lanes of a wave
 // diverging between the two kinds of counter.)
-constexpr int kHubSlots = 39808;
+constexpr int kHubSlots = 39552;
 // Phase B defers centres whose walk spans several windows (kECap positions with c3,
 // kWalkChunk without) to a task list shared by the workgroup's waves, up to kHeavyCap
 // per item; the hub counters above leave LDS room for that list (4 KB)
@@ -340,6 +341,10 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   __shared__ unsigned s_e[kPivWaves][kECap];
   __shared__ unsigned s_hub[kHub > 0 ? kHub : 1];
   __shared__ int s_item, s_next, s_nheavy, s_task, s_anext, s_anheavy, s_atask;
+  // per wave (launches with c3): the window-cut list of each lane's x in the flattened
+  // c3 scatter, its start and its offset in the sequence
+  __shared__ long long s_flb[kECap > 1 ? kPivWaves : 1][kWave];
+  __shared__ int s_foff[kECap > 1 ? kPivWaves : 1][kWave + 1];
   __shared__ int s_heavy[kHeavyCap], s_hwin[kHeavyCap];
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -347,7 +352,16 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   const G g{idx, adj};
   LaneAcc acc[NJ];
   // algorithmic global-memory bytes of this schedule: every logical load, store or
-  // atomic on a global array counted at its size (atomics twice), no cache credit
+  // atomic on a global array counted at its size (atomics twice), no cache credit; a
+  // binary search over a list of length L counts bitlen(L + 1) probes. The clearing of
+  // D (every 255 items, and at the end) is scratch upkeep that depends on how the
+  // dynamic queue spreads items, and is not counted. Per unit:
+  //   2-hop path 20 (adj, two atomics counted twice) or 4 (LDS hub counter);
+  //   adjacency mark 16, 20 in hub launches, 4 for an LDS hub; neighbour x of the
+  //   pivot 12 + 4 bitlen(d_x + 1); centre 16 + 4 bitlen(d_u + 1); walk position 16,
+  //   12 for an LDS hub; c3 entry x of T(u,w) per window 16 + s + (8 + s) bitlen(t_ux + 1)
+  //   (twice the search in a window that is not the centre's last); list element s
+  //   (s = 2 for 16-bit lists, else 4)
   ull bytes = 0;
   // D entries carry the item's epoch in their top 8 bits: an entry written for an
   // earlier item reads as zero, so no pass clears D after an item. Every 255 items,
@@ -382,7 +396,6 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     }
     if (++epoch == 256) {
       for (long long i = threadIdx.x; i < dn; i += kPivWaves * kWave) D[i] = 0u;
-      if (threadIdx.x == 0) bytes += 4 * dn;
       __syncthreads();
       epoch = 1;
     }
@@ -492,6 +505,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
 
     // one window [q_lo, q_hi) of centre u's walk past w: the c3 scatter into this
     // wave's LDS counters (when c3on), then the walk
+    // one window [q_lo, q_hi) of centre u's walk past w: the c3 scatter into this
+    // wave's LDS counters (when c3on), then the walk
     auto window = [&](int u, int ub, int du, int tuw, bool c3on, long long tb, long long te, int q_lo, int q_hi) {
       unsigned* __restrict__ E = s_e[wave];
       const bool last = q_hi == du;
@@ -499,85 +514,72 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // scatter: for x in T(u, w), for y in T(u, x) with q_lo <= y < q_hi: E[y - q_lo]++
+        // scatter: for x in T(u, w), for y in T(u, x) with q_lo <= y < q_hi: E[y - q_lo]++.
+        // Each lane cuts the list of its own x to the window; the cut lists of the 64
+        // lanes are then streamed as one sequence, eight elements per lane in flight.
+        long long* __restrict__ FLB = s_flb[kECap > 1 ? wave : 0];
+        int* __restrict__ FOFF = s_foff[kECap > 1 ? wave : 0];
         for (long long t0 = tb; t0 < te; t0 += kWave) {
           const long long t = t0 + lane;
-          int xq = -1;
-          if (t < te) xq = (int)tl[t];
-          // each lane walks its own list; long lists are shared by the wave
           long long lb = 0, le = 0;
-          if (xq >= 0) {
+          if (t < te) {
+            const int xq = (int)tl[t];
             const long long b0 = tl_off[ub + xq], e0 = tl_off[ub + xq + 1];
             long long lo = b0, hi = e0;  // first entry >= q_lo
             while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_lo) lo = mid + 1; else hi = mid; }
+            bytes += 16 + sizeof(TL) + (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1));
             lb = lo;
             le = e0;
             if (!last) {  // first entry >= q_hi
               hi = e0;
               while (lo < hi) { const long long mid = (lo + hi) >> 1; if ((int)tl[mid] < q_hi) lo = mid + 1; else hi = mid; }
               le = lo;
+              bytes += (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1));
             }
-            bytes += 16 + sizeof(TL) + (8 + sizeof(TL)) * (64 - __clzll(e0 - b0 + 1)) * (last ? 1 : 2) +
-                     (le - lb) * sizeof(TL);
+            bytes += (le - lb) * sizeof(TL);
           }
-          const bool big = (le - lb) > 64;
-          if (!big) {
-            long long k = lb;
-            for (; k + 4 <= le; k += 4) {
-              const int a0 = (int)tl[k], a1 = (int)tl[k + 1], a2 = (int)tl[k + 2], a3 = (int)tl[k + 3];
-              atomicAdd(&E[a0 - q_lo], 1u);
-              atomicAdd(&E[a1 - q_lo], 1u);
-              atomicAdd(&E[a2 - q_lo], 1u);
-              atomicAdd(&E[a3 - q_lo], 1u);
-            }
-            for (; k < le; k++) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
+          const int len = (int)(le - lb);
+          int incl = len;
+#pragma unroll
+          for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += v;
           }
-          ull bm = __ballot(big);
-          while (bm) {
-            const int who = __builtin_ctzll(bm);
-            bm &= bm - 1;
-            const long long B = bcast(lb, who), Ee = bcast(le, who);
-            long long k = B + lane;
-            if constexpr (sizeof(TL) == 2) {
-              // 16-bit lists: four positions per 8-byte load, four loads in flight per
-              // lane. The head up to the first multiple of 4 (8-byte aligned, as tl
-              // itself is) and the tail past the last full group go one per lane.
-              const long long a4 = min(Ee, (B + 3) & ~3ll);
-              if (B + lane < a4) atomicAdd(&E[(int)tl[B + lane] - q_lo], 1u);
-              const long long n4 = (Ee - a4) >> 2;
-              const ushort4* __restrict__ v4 = reinterpret_cast<const ushort4*>(tl + a4);
-              long long g4 = lane;
-              for (; g4 + 3 * kWave < n4; g4 += 4 * kWave) {
-                ushort4 r[4];
+          const int tot = __shfl(incl, kWave - 1, kWave);
+          __builtin_amdgcn_wave_barrier();  // the previous batch's readers of FLB / FOFF are done
+          FLB[lane] = lb;
+          FOFF[lane] = incl - len;
+          if (lane == kWave - 1) FOFF[kWave] = tot;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          // element g of the sequence is element g - FOFF[k] of lane k's cut list, k the
+          // last lane with FOFF[k] <= g (a lane with an empty list shares its offset with
+          // the next one, which wins); k only grows with g
+          int k = 0;
+          {
+            int lo = 0, hi = kWave;
+            const int g0 = min(lane, max(tot - 1, 0));
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (FOFF[mid] <= g0) lo = mid; else hi = mid;
+            }
+            k = lo;
+          }
+          for (int c = 0; c < tot; c += 8 * kWave) {
+            int a[8];
 #pragma unroll
-                for (int i = 0; i < 4; i++) r[i] = v4[g4 + i * kWave];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                  atomicAdd(&E[(int)r[i].x - q_lo], 1u);
-                  atomicAdd(&E[(int)r[i].y - q_lo], 1u);
-                  atomicAdd(&E[(int)r[i].z - q_lo], 1u);
-                  atomicAdd(&E[(int)r[i].w - q_lo], 1u);
-                }
-              }
-              for (; g4 < n4; g4 += kWave) {
-                const ushort4 r = v4[g4];
-                atomicAdd(&E[(int)r.x - q_lo], 1u);
-                atomicAdd(&E[(int)r.y - q_lo], 1u);
-                atomicAdd(&E[(int)r.z - q_lo], 1u);
-                atomicAdd(&E[(int)r.w - q_lo], 1u);
-              }
-              k = a4 + (n4 << 2) + lane;  // the tail, below
-            } else {
-              // eight independent loads in flight per lane before their increments
-              for (; k + 7 * kWave < Ee; k += 8 * kWave) {
-                int a[8];
-#pragma unroll
-                for (int r = 0; r < 8; r++) a[r] = (int)tl[k + r * kWave];
-#pragma unroll
-                for (int r = 0; r < 8; r++) atomicAdd(&E[a[r] - q_lo], 1u);
+            for (int r = 0; r < 8; r++) {
+              const int g = c + r * kWave + lane;
+              a[r] = -1;
+              if (g < tot) {
+                while (FOFF[k + 1] <= g) k++;
+                a[r] = (int)tl[FLB[k] + (g - FOFF[k])];
               }
             }
-            for (; k < Ee; k += kWave) atomicAdd(&E[(int)tl[k] - q_lo], 1u);
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+              if (a[r] >= 0) atomicAdd(&E[a[r] - q_lo], 1u);
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -691,7 +693,6 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
   // leave D zero for the next launch
   if (epoch > 0) {
     for (long long i = threadIdx.x; i < dn; i += kPivWaves * kWave) D[i] = 0u;
-    if (threadIdx.x == 0) bytes += 4 * dn;
   }
 #pragma unroll
   for (int j = 0; j < NJ; j++)
@@ -909,8 +910,10 @@ static int pivot_grid(const Tune& t)
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // one workgroup per CU for the 128 KB shapes (two with 64 KB), twice over
-  return t.grid > 0 ? t.grid : (pivot_waves(t) > 4 ? cus * 2 : cus * 4);
+  // as many workgroups as fit at once: one per CU for the 128 KB shapes, two with 64 KB
+  // (the queue is dynamic; 256, 512 and 768 workgroups of 16 waves time the same on
+  // R-MAT s20, and each holds a D array of n counters)
+  return t.grid > 0 ? t.grid : (pivot_waves(t) > 4 ? cus : cus * 2);
 }
 
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
@@ -918,6 +921,15 @@ size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
   const size_t grid = (size_t)pivot_grid(t);
   const size_t dstride = ((size_t)g.n + 63) / 64 * 64;
   return sizeof(unsigned) * grid * dstride + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
+}
+
+// The cost above which a pivot's centres are split over several work items: 1/4096 of
+// the launch's total (16 items per CU of an MI355X), at least 2^16 and chunk * 4096.
+// It does not depend on the device, so the items, and the algorithmic bytes, are a
+// function of the graph alone.
+long long pivot_split_budget(long long total_cost, int chunk)
+{
+  return std::max<long long>(std::max<long long>(1 << 16, (long long)chunk << 12), total_cost / 4096);
 }
 
 // Prepares one pivot-schedule launch of up to kMaxPivotJobs wedge-class jobs.
@@ -946,8 +958,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   // order then balances the rest.
   long long total_cost = 0;
   for (int w = 0; w < g->n; w++) total_cost += g->h_piv_cost[w];
-  const long long budget = std::max<long long>(std::max<long long>(1 << 16, (long long)t.chunk << 12),
-                                               total_cost / ((long long)pivot_grid(t) * 8));
+  const long long budget = pivot_split_budget(total_cost, t.chunk);
   std::vector<dev::PivotItem> items;
   std::vector<long long> icost;
   for (int w = 0; w < g->n; w++) {

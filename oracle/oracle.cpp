@@ -925,3 +925,111 @@ void oracle_ncr(int n, int k, u64* lo, u64* hi, uint32_t* flags)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- pivot schedule bytes
+//
+// Not a restatement of the reference: the unit counts of the library's pivot schedule
+// for wedge cores (its algorithmic-bytes model), computed from the CSR alone, so that
+// the bytes a pivot launch reports can be recomputed from units. The schedule's work
+// items (pivot w with a range of its centres), its c3 windows and its LDS hubs follow
+// the library's rules:
+//   items: pivot w of cost c = sum of its neighbours' degrees is cut into
+//          k = max(1, min(d_w, c / max(budget, c / 8 + 1))) centre ranges; every item
+//          repeats the pivot's 2-hop scatter (phase A);
+//   windows: a centre u of pivot w (w at position rq of N(u)) walks positions rq+1..d_u-1,
+//          with c3 (a launch that needs it and t(u,w) > 0) in ceil((d_u-rq-1)/ecap) windows;
+//   hubs: in launches without c3, the hub_slots vertices of highest degree (ties by the
+//          smaller id) have LDS counters.
+// Units (out[0..12]): 2-hop paths to D, to LDS hubs; adjacency marks to D, to LDS hubs;
+// pivot neighbours visited and the sum of their bitlen(d_x + 1); centres and the sum of
+// bitlen(d_u + 1); c3 entries (x of T(u,w), per window) and the sum of their search
+// lengths bitlen(t_ux + 1) (counted twice in a window that is not the centre's last);
+// list elements scattered (= sum over directed edges of C(t_e, 2)); walk positions to
+// D, to LDS hubs.
+static int bitlen(long long x) { return 64 - __builtin_clzll((unsigned long long)x); }
+
+extern "C" int oracle_pivot_units(int n, long long m, const int* idx, const int* adj, int ecap, int hub_slots,
+                                  long long budget, int c3, int threads, u64* out)
+{
+  G g{n, m, idx, adj};
+  if (threads > 0) omp_set_num_threads(threads);
+  // supports t_e per directed edge (merge intersection)
+  std::vector<int> tsup((size_t)std::max<long long>(1, m));
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int u = 0; u < n; u++) {
+    for (int e = g.beg(u); e < g.end(u); e++) {
+      const int x = adj[e];
+      int i = g.beg(u), j = g.beg(x), c = 0;
+      while (i < g.end(u) && j < g.end(x)) {
+        if (adj[i] < adj[j]) i++;
+        else if (adj[i] > adj[j]) j++;
+        else { c++; i++; j++; }
+      }
+      tsup[e] = c;
+    }
+  }
+  std::vector<char> hub((size_t)n, 0);
+  if (hub_slots > 0) {
+    std::vector<int> ord(n);
+    for (int v = 0; v < n; v++) ord[v] = v;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return g.deg(a) > g.deg(b); });
+    for (int r = 0; r < std::min(n, hub_slots); r++) hub[ord[r]] = 1;
+  }
+  u64 U[13] = {0};
+#pragma omp parallel
+  {
+    u64 L[13] = {0};
+#pragma omp for schedule(dynamic, 64)
+    for (int w = 0; w < n; w++) {
+      const int dw = g.deg(w);
+      if (dw == 0) continue;
+      long long cost = 0;
+      for (int e = g.beg(w); e < g.end(w); e++) cost += g.deg(adj[e]);
+      const long long k = std::max<long long>(1, std::min<long long>(dw, cost / std::max(budget, cost / 8 + 1)));
+      // phase A, once per item
+      u64 pd = 0, ph = 0, md = 0, mh = 0, xb = 0;
+      for (int e = g.beg(w); e < g.end(w); e++) {
+        const int x = adj[e];
+        xb += bitlen((long long)g.deg(x) + 1);
+        if (x > w) { if (hub[x]) mh++; else md++; }
+        const int s = (int)(std::upper_bound(adj + g.beg(x), adj + g.end(x), w) - adj);
+        for (int i = s; i < g.end(x); i++) { if (hub[adj[i]]) ph++; else pd++; }
+      }
+      long long items = 0;
+      for (long long s = 0; s < k; s++) if (dw * (s + 1) / k > dw * s / k) items++;
+      L[0] += pd * items; L[1] += ph * items; L[2] += md * items; L[3] += mh * items;
+      L[4] += (u64)dw * items; L[5] += xb * items;
+      // phase B: every centre once
+      for (int e = g.beg(w); e < g.end(w); e++) {
+        const int u = adj[e];
+        const int ub = g.beg(u), du = g.deg(u);
+        const int rq = (int)(std::lower_bound(adj + ub, adj + ub + du, w) - (adj + ub));
+        if (rq + 1 >= du) continue;
+        L[6]++;
+        L[7] += bitlen((long long)du + 1);
+        for (int q = rq + 1; q < du; q++) { if (hub[adj[ub + q]]) L[12]++; else L[11]++; }
+        const int tuw = tsup[(size_t)ub + rq];
+        if (!c3 || tuw == 0) continue;
+        const long long nwin = (du - rq - 1 + ecap - 1) / ecap;
+        // x in T(u,w): common neighbours of u and w, with t_ux
+        int i = ub, j = g.beg(w);
+        while (i < g.end(u) && j < g.end(w)) {
+          if (adj[i] < adj[j]) i++;
+          else if (adj[i] > adj[j]) j++;
+          else {
+            const int b = bitlen((long long)tsup[i] + 1);
+            L[8] += nwin;
+            L[9] += (u64)b * (u64)(2 * nwin - 1);
+            i++; j++;
+          }
+        }
+      }
+    }
+#pragma omp critical
+    for (int q = 0; q < 13; q++) U[q] += L[q];
+  }
+  if (c3)
+    for (long long e = 0; e < m; e++) U[10] += (u64)tsup[e] * (u64)(tsup[e] - 1) / 2;
+  for (int q = 0; q < 13; q++) out[q] = U[q];
+  return 0;
+}

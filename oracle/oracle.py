@@ -42,6 +42,8 @@ def _lib():
     lib.oracle_functionx.argtypes = [C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_uint64),
                                      P(C.c_uint64), P(C.c_uint32)]
     lib.oracle_ncr.argtypes = [C.c_int, C.c_int, P(C.c_uint64), P(C.c_uint64), P(C.c_uint32)]
+    lib.oracle_pivot_units.argtypes = [C.c_int, C.c_longlong, P(C.c_int), P(C.c_int), C.c_int, C.c_int, C.c_longlong,
+                                       C.c_int, C.c_int, P(C.c_uint64)]
     return lib
 
 
@@ -125,3 +127,36 @@ def motif_graph_edges(mo: bytes):
                     edges.append((j, v))
             v += 1
     return v, edges
+
+
+PIVOT_UNITS = ("paths_d", "paths_hub", "marks_d", "marks_hub", "xvisits", "xvisit_bits", "centres", "centre_bits",
+               "c3_x", "c3_search_bits", "list_elems", "walk_d", "walk_hub")
+
+
+def pivot_split_budget(total_cost: int, chunk: int = 256) -> int:
+    """The library's pivot work-item split threshold (pivot.hip pivot_split_budget)."""
+    return max(1 << 16, chunk << 12, total_cost // 4096)
+
+
+def pivot_units(nindex, nlist, ecap: int, hub_slots: int, c3: bool, chunk: int = 256, threads: int = 0) -> dict:
+    """Unit counts of one pivot-schedule launch on this CSR (oracle.cpp oracle_pivot_units)."""
+    ni = np.ascontiguousarray(nindex, dtype=np.int32)
+    nl = np.ascontiguousarray(nlist, dtype=np.int32)
+    deg = np.diff(ni).astype(np.int64)
+    budget = pivot_split_budget(int((deg * deg).sum()), chunk)
+    out = (C.c_uint64 * 13)()
+    rc = lib().oracle_pivot_units(len(ni) - 1, len(nl), _ip(ni), _ip(nl), ecap, hub_slots, budget, int(c3), threads,
+                                  out)
+    if rc != 0:
+        raise RuntimeError("oracle_pivot_units failed")
+    return dict(zip(PIVOT_UNITS, (int(x) for x in out)))
+
+
+def pivot_bytes(units: dict, list_elem_bytes: int, hub_launch: bool) -> int:
+    """The pivot kernel's algorithmic bytes from its units (the per-unit sizes of
+    pivot.hip k_wedge_pivot)."""
+    s = list_elem_bytes
+    return (20 * units["paths_d"] + 4 * units["paths_hub"] + (20 if hub_launch else 16) * units["marks_d"]
+            + 4 * units["marks_hub"] + 12 * units["xvisits"] + 4 * units["xvisit_bits"] + 16 * units["centres"]
+            + 4 * units["centre_bits"] + (16 + s) * units["c3_x"] + (8 + s) * units["c3_search_bits"]
+            + s * units["list_elems"] + 16 * units["walk_d"] + 12 * units["walk_hub"])
