@@ -1,6 +1,6 @@
 // Autotuner: per graph and schedule, the launch shape (workgroup size where the kernel
-// has several, workgroups, roots per work item, hub threshold) with the shortest
-// device time on a sample of the work items.
+// has several, workgroups, roots per work item, hub threshold, LDS staging cap) with
+// the shortest device time on a sample of the work items.
 // The counts never depend on the shape (every shape enumerates the same roots), so
 // tuning only changes time.
 #include <hip/hip_runtime.h>
@@ -15,16 +15,18 @@
 namespace {
 
 struct Cand {
-  int block, grid, chunk, hub;
+  int block, grid, chunk, hub, lds;
 };
 
 std::vector<Cand> candidates(int sched, int cus)
 {
   std::vector<Cand> c;
-  std::vector<int> grids, chunks = {64, 256, 1024}, hubs = {4096}, blocks = {256};
+  std::vector<int> grids, chunks = {64, 256, 1024}, hubs = {4096}, blocks = {256}, lds = {0};
   switch (sched) {
-    case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 256}; break;
-    case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; break;
+    // the LDS cap shortens the pivot kernel's c3 windows (more windows, shorter LDS
+    // clears) and sends more of the rank kernel's vertices to global marks
+    case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 256}; lds = {0, 1024}; break;
+    case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; lds = {0, 2048}; break;
     case fsgc::kEdgePass: grids = {cus * 4, cus * 8, cus * 16}; break;
     default: grids = {cus * 4, cus * 8, cus * 16}; hubs = {1024, 4096}; break;
   }
@@ -32,7 +34,8 @@ std::vector<Cand> candidates(int sched, int cus)
     for (int g : grids) {
       if (b > 256 && g > cus * 2) continue;  // one 1024-thread workgroup per CU fits; more only queue
       for (int ch : chunks)
-        for (int h : hubs) c.push_back(Cand{b, g, ch, h});
+        for (int h : hubs)
+          for (int l : lds) c.push_back(Cand{b, g, ch, h, l});
     }
   return c;
 }
@@ -80,7 +83,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     fsgc_tune def;
     std::memset(&def, 0, sizeof(def));
     def.block = s == fsgc::kPivot ? 1024 : 256;
-    def.grid = s == fsgc::kPivot || s == fsgc::kRank ? cus * 4 : cus * 8;
+    def.grid = s == fsgc::kPivot ? cus : (s == fsgc::kRank ? cus * 4 : cus * 8);
     def.chunk = d.chunk;
     def.hub_degree = d.hub_degree;
     def.lds_cap = d.lds_cap;
@@ -98,15 +101,16 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
       ft.grid = c.grid;
       ft.chunk = c.chunk;
       ft.hub_degree = c.hub;
+      ft.lds_cap = c.lds;
       fsgc::Tune t;
-      t.block = c.block; t.grid = c.grid; t.chunk = c.chunk; t.hub_degree = c.hub;
+      t.block = c.block; t.grid = c.grid; t.chunk = c.chunk; t.hub_degree = c.hub; t.lds_cap = c.lds;
       if (fsgc::scratch_for(g, s, t) > free_b / 4) continue;  // leave room for everything else
       rc = time_shape(ft, &ms);
       if (rc) { fsgc::set_error(err); return rc; }
       ncand++;
       if (fsgc::log_on())
-        fprintf(stderr, "[fsgc] tune %s block %d grid %d chunk %d hub %d: %.2f ms\n", fsgc::schedule_name(s), c.block,
-                c.grid, c.chunk, c.hub, ms);
+        fprintf(stderr, "[fsgc] tune %s block %d grid %d chunk %d hub %d lds %d: %.2f ms\n", fsgc::schedule_name(s),
+                c.block, c.grid, c.chunk, c.hub, c.lds, ms);
       if (ms < best_ms) { best_ms = ms; best = ft; }
     }
     {

@@ -286,7 +286,7 @@ class Graph:
         for r in reps[: nr.value]:
             out.append({"schedule": SCHEDULE_NAMES[r.schedule], "candidates": r.candidates,
                         "block": r.best.block, "grid": r.best.grid, "chunk": r.best.chunk, "hub_degree": r.best.hub_degree,
-                        "best_ms": r.best_ms, "default_ms": r.default_ms})
+                        "lds_cap": r.best.lds_cap, "best_ms": r.best_ms, "default_ms": r.default_ms})
         return out
 
     def untune(self):
