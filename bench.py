@@ -90,45 +90,67 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True, quick=False):
+def _result_fields(r):
+    return {"class": r.class_name, "exact": str(r.exact), "sg": str(r.sg), "aut": r.aut, "ref_compat": r.ref_compat,
+            "core_matches": r.core_matches, "flags": r.flags}
+
+
+def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True, quick=False, part="both",
+                  relabel="none"):
     """Plain path (one fsgc_count per motif) against the batched path (one
     fsgc_count_batch over all motifs) on the same resident graph: motifs/s of each,
     per-class times, and per-motif equality of every result. `classes` restricts the
-    motif set to the named classes (the others are listed as excluded)."""
+    motif set to the named classes (the others are listed as excluded). `part` runs
+    only the plain or only the batched side ("plain" / "batched"), for graphs where
+    both do not fit one run; batch_merge() then compares two such records. `relabel`
+    counts on the graph renumbered by degree (counts are label-invariant)."""
     import fsgc
     fsgc.init(0)
     t0 = time.perf_counter()
     g = fsgc.Graph.generate(spec)
+    relabel_ms = None
+    if relabel != "none":
+        g0 = g
+        g, _, relabel_ms = g0.relabel(relabel)
+        g0.free()
     n, m, maxdeg = g.info()
-    print(f"# graph {spec}: n={n} m={m} maxdeg={maxdeg} ({time.perf_counter() - t0:.1f} s)", flush=True)
+    print(f"# graph {spec}: n={n} m={m} maxdeg={maxdeg} relabel={relabel} ({time.perf_counter() - t0:.1f} s)",
+          flush=True)
     mos_all = fsgc.random_motifs(seed, count)
     cls_of = {i: fsgc.motif_class(mo) for i, mo in enumerate(mos_all)}
     keep = [i for i in range(count) if classes is None or cls_of[i] in classes]
     mos = [mos_all[i] for i in keep]
-    # automorphism sums are per motif and the per-edge index per graph: warm them once
+    # the automorphism sums (per motif, always whole) and the per-graph index are built
+    # once, by a batch over a thin root slice, so neither side below pays for them
     t0 = time.perf_counter()
-    g.count_batch(mos)
-    print(f"# warm batch (index + automorphism sums) {time.perf_counter() - t0:.1f} s", flush=True)
-    plain, t_plain, per_class = [], 0.0, {}
-    for i, mo in zip(keep, mos):
+    g.count_batch(mos, fsgc.Tune(slice_mod=1 << 20, slice_rem=1))
+    print(f"# warm batch (index + automorphism sums, root slice u % 2^20 == 1) {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    plain, t_plain, per_class, plain_secs = [], 0.0, {}, []
+    if part in ("both", "plain"):
+        for i, mo in zip(keep, mos):
+            t0 = time.perf_counter()
+            r = g.count(mo)
+            dt = time.perf_counter() - t0
+            t_plain += dt
+            plain.append(r)
+            plain_secs.append(round(dt, 4))
+            pc = per_class.setdefault(r.class_name, {"motifs": 0, "plain_s": 0.0})
+            pc["motifs"] += 1
+            pc["plain_s"] += dt
+            print(f"# plain motif {i} ({r.class_name}): {dt:.3f} s {json.dumps(_result_fields(r))}", flush=True)
+    batch, t_batch = [], None
+    if part in ("both", "batched"):
         t0 = time.perf_counter()
-        r = g.count(mo)
-        dt = time.perf_counter() - t0
-        t_plain += dt
-        plain.append(r)
-        pc = per_class.setdefault(r.class_name, {"motifs": 0, "plain_s": 0.0})
-        pc["motifs"] += 1
-        pc["plain_s"] += dt
-        print(f"# plain motif {i} ({r.class_name}): {dt:.3f} s", flush=True)
-    t0 = time.perf_counter()
-    batch, batch_ms = g.count_batch(mos)
-    t_batch = time.perf_counter() - t0
-    print(f"# batched: {t_batch:.3f} s", flush=True)
-    same = all((a.exact, a.sg, a.aut, a.ref_compat, a.core_matches, a.flags) ==
-               (b.exact, b.sg, b.aut, b.ref_compat, b.core_matches, b.flags) for a, b in zip(plain, batch))
+        batch, batch_ms = g.count_batch(mos)
+        t_batch = time.perf_counter() - t0
+        print(f"# batched: {t_batch:.3f} s", flush=True)
+    same = None
+    if part == "both":
+        same = all(_result_fields(a) == _result_fields(b) for a, b in zip(plain, batch))
     # the same batch captured once as a HIP graph, then replayed
     t_cap, same_cap = None, None
-    if not quick:
+    if not quick and part == "both":
         cb = g.capture_batch(mos)
         t0 = time.perf_counter()
         cap, cap_ms = cb.run()
@@ -137,32 +159,67 @@ def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True
         same_cap = all((a.exact, a.sg, a.core_matches) == (b.exact, b.sg, b.core_matches) for a, b in zip(plain, cap))
         print(f"# captured batch replay: {t_cap:.3f} s, identical {same_cap}", flush=True)
     # the same per class (one count_batch per class)
-    for c, pc in (per_class.items() if not quick else []):
+    for c, pc in (per_class.items() if not quick and part == "both" else []):
         sub = [mo for mo, r in zip(mos, plain) if r.class_name == c]
         t0 = time.perf_counter()
         g.count_batch(sub)
         pc["batched_s"] = time.perf_counter() - t0
         pc["speedup"] = round(pc["plain_s"] / pc["batched_s"], 3)
-        pc["plain_s"] = round(pc["plain_s"], 4)
-        pc["batched_s"] = round(pc["batched_s"], 4)
         print(f"# class {c}: {pc}", flush=True)
+    for pc in per_class.values():
+        for k in ("plain_s", "batched_s"):
+            if k in pc:
+                pc[k] = round(pc[k], 4)
     excluded = {}
     for i in range(count):
         if i not in keep:
             excluded[cls_of[i]] = excluded.get(cls_of[i], 0) + 1
     res = {
-        "graph": spec, "nodes": n, "directed_edges": m, "maxdeg": maxdeg,
+        "graph": spec, "nodes": n, "directed_edges": m, "maxdeg": maxdeg, "relabel": relabel,
+        "relabel_ms": round(relabel_ms, 1) if relabel_ms is not None else None, "part": part,
         "motifs": len(mos), "motifs_generated": count, "seed": seed, "excluded": excluded,
-        "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
-        "plain_motifs_per_s": round(len(mos) / t_plain, 4), "batched_motifs_per_s": round(len(mos) / t_batch, 4),
-        "speedup": round(t_plain / t_batch, 3), "identical": same,
+        "motif_ids": keep,
+        "plain_s": round(t_plain, 3) if plain else None, "batched_s": round(t_batch, 3) if batch else None,
+        "plain_motifs_per_s": round(len(mos) / t_plain, 4) if plain else None,
+        "batched_motifs_per_s": round(len(mos) / t_batch, 4) if batch else None,
+        "speedup": round(t_plain / t_batch, 3) if plain and batch else None, "identical": same,
         "captured_s": round(t_cap, 3) if t_cap else None,
         "captured_speedup": round(t_plain / t_cap, 3) if t_cap else None, "captured_identical": same_cap,
         "per_class": per_class,
     }
     print(json.dumps(res) if emit else "# " + json.dumps(res), flush=True)
     if out_path:
-        json.dump({"summary": res, "counts": [str(r.exact) for r in batch]}, open(out_path, "w"), indent=1)
+        rows = batch if batch else plain
+        rec = {"summary": res, "counts": [str(r.exact) for r in rows]}
+        if plain:
+            rec["plain_results"] = [_result_fields(r) for r in plain]
+            rec["plain_seconds_per_motif"] = plain_secs
+        if batch:
+            rec["batched_results"] = [_result_fields(r) for r in batch]
+        json.dump(rec, open(out_path, "w"), indent=1)
+    return res
+
+
+def batch_merge(paths, out_path=None):
+    """Combines a plain-only and a batched-only record of batch_compare (same graph,
+    seed and motif set) into one comparison: times, speedup and per-motif equality of
+    every result field."""
+    recs = [json.load(open(p)) for p in paths]
+    pl = next(r for r in recs if "plain_results" in r)
+    bt = next(r for r in recs if "batched_results" in r)
+    a, b = pl["summary"], bt["summary"]
+    for k in ("graph", "seed", "motifs_generated", "motif_ids"):
+        if a.get(k) != b.get(k):
+            raise SystemExit(f"batch_merge: records differ in {k}: {a.get(k)} vs {b.get(k)}")
+    same = pl["plain_results"] == bt["batched_results"]
+    res = dict(a)
+    res.update({"part": "merged", "batched_s": b["batched_s"], "batched_motifs_per_s": b["batched_motifs_per_s"],
+                "speedup": round(a["plain_s"] / b["batched_s"], 3), "identical": same,
+                "relabel_batched": b.get("relabel"), "runs": [os.path.basename(p) for p in paths]})
+    print(json.dumps(res), flush=True)
+    if out_path:
+        json.dump({"summary": res, "counts": bt["counts"], "plain_results": pl["plain_results"],
+                   "batched_results": bt["batched_results"]}, open(out_path, "w"), indent=1)
     return res
 
 
@@ -282,6 +339,11 @@ def main():
     ap.add_argument("--batch", default=None, help="graph spec: compare plain vs batched over the random motifs")
     ap.add_argument("--batch-out", default=None)
     ap.add_argument("--batch-quick", action="store_true", help="--batch: skip the captured replay and per-class runs")
+    ap.add_argument("--batch-part", default="both", choices=["both", "plain", "batched"],
+                    help="--batch: run only one side (merge two such records with --batch-merge)")
+    ap.add_argument("--batch-relabel", default="none", choices=["none", "asc", "desc"],
+                    help="--batch: count on the graph renumbered by degree")
+    ap.add_argument("--batch-merge", default=None, help="comma-separated plain-only and batched-only --batch-out files")
     ap.add_argument("--classes", default=None, help="comma-separated motif classes for --batch / --serve")
     ap.add_argument("--serve", default=None, help="comma-separated graph specs: served Poisson query stream")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve: offered queries per second")
@@ -294,8 +356,12 @@ def main():
                          "region, and is reported as relabel_ms")
     args = ap.parse_args()
     classes = set(args.classes.split(",")) if args.classes else None
+    if args.batch_merge:
+        batch_merge(args.batch_merge.split(","), out_path=args.batch_out)
+        return
     if args.batch:
-        batch_compare(args.batch, classes=classes, out_path=args.batch_out, quick=args.batch_quick)
+        batch_compare(args.batch, classes=classes, out_path=args.batch_out, quick=args.batch_quick,
+                      part=args.batch_part, relabel=args.batch_relabel)
         return
     if args.serve:
         serve_bench(args.serve.split(","), args.rate, args.queries, args.slots, classes=classes,

@@ -27,6 +27,7 @@ const char* schedule_name(int s)
     case kPivot: return "pivot";
     case kRank: return "rank";
     case kEdgePass: return "edge-pass";
+    case kClique: return "clique";
     default: return "?";
   }
 }
@@ -99,7 +100,9 @@ void finish(const Prepared& p, const JobResult& r, int32_t n, int64_t m, fsgc_re
 
 // The pivot schedule for wedge cores, the rank schedule for triangle cores and the
 // triangle kinds of the edge core, one edge pass for the other edge kinds and the
-// stars, the per-root persistent kernel otherwise (or for everything when !fast).
+// stars, the clique schedule for 4-clique cores whose fringe counts it evaluates
+// exactly on this graph, the per-root persistent kernel otherwise (or for everything
+// when !fast).
 // Wedge cores on a graph whose maximum degree exceeds kPivotMaxDegree take the
 // per-root kernel: the pivot kernel's 24-bit 2-hop counters could not hold a codegree.
 Schedule schedule_of(const Prepared& p, bool fast, int32_t maxdeg)
@@ -108,6 +111,7 @@ Schedule schedule_of(const Prepared& p, bool fast, int32_t maxdeg)
   if (p.job.cls == kWedge) return maxdeg <= kPivotMaxDegree ? kPivot : kPersistent;
   if (triangle_schedule_ok(p.job)) return kRank;
   if (edgepass_schedule_ok(p.job)) return kEdgePass;
+  if (clique_fits(p.job, maxdeg)) return kClique;
   return kPersistent;
 }
 
@@ -133,6 +137,7 @@ size_t scratch_for(const fsgc_graph* g, int sched, const Tune& t)
     case kPivot: return pivot_scratch_bytes(g->dg, t);
     case kRank: return triangle_scratch_bytes(g->dg, t);
     case kEdgePass: return edgepass_scratch_bytes(g->dg, t);
+    case kClique: return clique_scratch_bytes(g->dg, t);
     default: return perroot_scratch_bytes(g->dg, t);
   }
 }
@@ -172,7 +177,7 @@ std::vector<Group> make_groups(const std::vector<Prepared>& ps, bool fast, int32
   if (!perroot.idx.empty()) out.push_back(perroot);
   if (!fast) return out;
   // each fast-schedule launch takes up to 8 motifs that share one enumeration
-  for (int sched : {kPivot, kRank, kEdgePass}) {
+  for (int sched : {kPivot, kRank, kEdgePass, kClique}) {
     std::vector<Group> groups;
     for (size_t i = 0; i < ps.size(); i++) {
       if (ps[i].trivial || schedule_of(ps[i], fast, maxdeg) != sched) continue;
@@ -210,6 +215,7 @@ int plan_group(fsgc_graph* g, const std::vector<Prepared>& ps, const Group& gr, 
     case kPivot: return plan_wedge_pivot(jobs, &g->dg, t, stream, lp, err);
     case kRank: return plan_triangles(jobs, &g->dg, t, stream, lp, err);
     case kEdgePass: return plan_edgepass(jobs, &g->dg, t, stream, lp, err);
+    case kClique: return plan_cliques(jobs, &g->dg, t, stream, lp, err);
     default: {
       std::vector<const DeviceGraph*> graphs(jobs.size(), &g->dg);
       return plan_jobs(jobs, graphs, t, stream, lp, err);

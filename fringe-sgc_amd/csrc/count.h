@@ -12,7 +12,7 @@
 namespace fsgc {
 
 // Which kernel counts a motif on the data graph.
-enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2, kEdgePass = 3, kNumSchedules = 4 };
+enum Schedule { kPersistent = 0, kPivot = 1, kRank = 2, kEdgePass = 3, kClique = 4, kNumSchedules = 5 };
 const char* schedule_name(int s);
 
 // Automorphism sum of a motif (run on the motif graph, cached by the live fields).
@@ -41,7 +41,7 @@ struct Prepared {
 struct fsgc_graph {
   fsgc::DeviceGraph dg;
   std::mutex tune_mu;
-  bool tuned[fsgc::kNumSchedules] = {false, false, false, false};
+  bool tuned[fsgc::kNumSchedules] = {false, false, false, false, false};
   fsgc::Tune best[fsgc::kNumSchedules];
 };
 

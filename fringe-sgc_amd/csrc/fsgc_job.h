@@ -82,6 +82,43 @@ struct JobDesc {
   Accum* out;
 };
 
+// 4-clique core (general core whose four vertices are pairwise adjacent), counted per
+// clique: the fringe count of one ordered embedding is an inclusion-exclusion over
+// the ways the labelled fringes can coincide,
+//   F * den = sum_t coef_t * prod_f N[mask_tf],
+// where N[U] is the number of non-core vertices adjacent to every clique vertex in U
+// and den = prod over anchor sets of fcnt!. A term packs its factor count in bits
+// 28..31 and up to seven 4-bit clique-vertex masks below.
+struct CliqueTerm {
+  int32_t coef;
+  uint32_t packed;
+};
+
+constexpr int kCliqueMaxTerms = 8192;  // terms of all assignment classes of one motif
+constexpr int kCliqueMaxFringes = 7;
+
+// The 24 assignments of the clique's vertices (ordered by rank) to the motif's core
+// positions, grouped into classes with equal fringe counts (assignments related by a
+// symmetry of the anchored motif); per class the terms of its first assignment.
+struct CliqueTab {
+  int32_t thr[4];          // minimum degree per core position (startdeg, destdeg[0..2])
+  int32_t ncls;
+  int32_t nterm;
+  uint32_t cls_perms[24];  // per class: bit i set for assignment i
+  uint8_t perm[24][4];     // assignment i: core position -> clique vertex (0..3)
+  int32_t term_off[25];    // terms of class c: [term_off[c], term_off[c + 1])
+  int32_t den_shift;       // den = 2^den_shift * odd
+  uint64_t den_inv_lo;     // inverse of odd modulo 2^128
+  uint64_t den_inv_hi;
+  // anchor sets with at least three fringes: the value of N at which the reference's
+  // u64 binomial of that many fringes wraps (kRefNckInexact), and per class the
+  // set's clique-vertex mask
+  int32_t nck_n;
+  int32_t nck_thr[8];
+  uint8_t nck_mask[24][8];
+  const CliqueTerm* terms;  // device copy of the nterm terms
+};
+
 // One unit of the device work queue: a range of roots of one job.
 // Roots are vertices for the vertex class and CSR positions otherwise.
 struct WorkItem {

@@ -17,6 +17,12 @@ int general_quirk_factor(const MatchingOrder& mo);
 int triangle_rotations(const int f[8], int fr[6][8]);
 int wedge_rotations(const int f[8], int fr[6][8]);
 HostGraph motif_graph(const MatchingOrder& mo);
+// 4-clique cores (general cores whose four vertices are pairwise adjacent)
+bool clique_core(const JobDesc& j);
+int job_fringes(const JobDesc& j);
+// whether the clique schedule counts this job exactly on a graph of maximum degree maxdeg
+bool clique_fits(const JobDesc& j, int32_t maxdeg);
+int build_clique_tab(const JobDesc& j, CliqueTab* tab, std::vector<CliqueTerm>* terms, std::string* err);
 
 // Launch parameters of the persistent counting kernel.
 struct Tune {
@@ -56,6 +62,8 @@ struct DeviceGraph {
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
   uint16_t* tl16 = nullptr;     // the same lists in 16 bits (built when maxdeg <= 65536; pivot kernel)
+  int32_t* c3tl = nullptr;      // parallel to tl: entry of c in T(a, b), a < b < c by rank: |N(a) & N(b) & N(c)|
+                                // (built for the clique schedule)
   long long tl_total = 0;
   long long* piv_cost = nullptr;         // n: sum of neighbour degrees
   std::vector<long long> h_piv_cost;
@@ -133,6 +141,14 @@ int device_run_edgepass(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& 
                         std::vector<JobResult>* out, double* kernel_ms, std::string* err);
 int device_run_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream,
                          std::vector<JobResult>* out, double* kernel_ms, std::string* err);
+// the triple counts c3tl of every triangle, once per graph (after the lists)
+int device_c3tl_build(DeviceGraph* g, void* stream, double* ms, std::string* err);
+
+// clique.hip
+constexpr int kMaxCliqueJobs = 8;
+size_t clique_scratch_bytes(const DeviceGraph& g, const Tune& t);
+int plan_cliques(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void* stream, LaunchPlan* lp,
+                 std::string* err);
 // One edge-pass job (closed-form edge core or stars) with all its device scratch in a
 // fixed array (no allocation); the graph's supports must be built. kernel_ms: host
 // time of the device part.

@@ -755,6 +755,8 @@ void device_derived_free(DeviceGraph* g)
   if (g->tl) (void)hipFree(g->tl);
   if (g->tl16) (void)hipFree(g->tl16);
   if (g->piv_cost) (void)hipFree(g->piv_cost);
+  if (g->c3tl) (void)hipFree(g->c3tl);
+  g->c3tl = nullptr;
   g->tsup = nullptr;
   g->adjdeg = nullptr;
   g->adjhub = nullptr;

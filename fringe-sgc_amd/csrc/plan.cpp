@@ -4,7 +4,10 @@
 #include "fsgc_plan.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
+#include <functional>
+#include <map>
 
 namespace fsgc {
 
@@ -240,6 +243,183 @@ int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
       break;
     }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------- 4-clique cores
+
+bool clique_core(const JobDesc& j)
+{
+  return j.cls == kGeneral && j.p[0] == 4 && j.p[7] == 1 && j.p[8] == 3 && j.p[9] == 7;
+}
+
+int job_fringes(const JobDesc& j)
+{
+  int s = 0;
+  for (int m = 1; m < 64; m++) s += j.fcnt_by_mask[m];
+  return s;
+}
+
+// The clique schedule evaluates F * den modulo 2^128 and divides exactly, which is
+// exact when den * F < 2^128. den * F <= prod_S N_S^{f_S} <= maxdeg^fringes; three
+// factors of N < 2^21 are multiplied in 64 bits.
+bool clique_fits(const JobDesc& j, int32_t maxdeg)
+{
+  if (!clique_core(j)) return false;
+  const int f = job_fringes(j);
+  if (f > kCliqueMaxFringes || maxdeg < 0 || maxdeg >= (1 << 21)) return false;
+  u128 p = 1;
+  for (int i = 0; i < f; i++) {
+    if (maxdeg > 0 && p > (~(u128)0) / (u128)(unsigned)maxdeg) return false;
+    p *= (u128)(unsigned)maxdeg;
+  }
+  CliqueTab tab;
+  std::vector<CliqueTerm> terms;
+  return build_clique_tab(j, &tab, &terms, nullptr) == 0;
+}
+
+// Tables of a 4-clique core. The fringes are labelled items; fringe q must go to a
+// non-core vertex adjacent to every core vertex of its anchor set S_q, all distinct.
+// By Moebius inversion over the set partitions P of the items, the number of such
+// injective placements is
+//   sum_P prod_{B in P} (-1)^{|B|-1} (|B|-1)! * N[union of the S_q, q in B],
+// and the count of fringe sets is that divided by prod fcnt! (the items of one set are
+// interchangeable). This is the quantity functionX enumerates (FC:408-486) for a core
+// whose Venn regions are all non-negative, as every clique's are. Terms with the
+// same multiset of unions are merged. Each of the 24 assignments of the clique's
+// vertices to the core positions maps the unions to clique-vertex masks; assignments
+// giving the same mapped term list form one class.
+int build_clique_tab(const JobDesc& j, CliqueTab* tab, std::vector<CliqueTerm>* terms, std::string* err)
+{
+  if (!clique_core(j)) { if (err) *err = "not a 4-clique core"; return -22; }
+  std::vector<int> lab;  // anchor mask (core positions) of each labelled fringe
+  std::vector<std::pair<int, int>> sets;  // (mask, fringes)
+  u128 den = 1;
+  for (int m = 1; m < 16; m++) {
+    const int f = j.fcnt_by_mask[m];
+    if (f <= 0) continue;
+    sets.push_back({m, f});
+    for (int q = 0; q < f; q++) { lab.push_back(m); den *= (u128)(q + 1); }
+  }
+  for (int m = 16; m < 64; m++)
+    if (j.fcnt_by_mask[m]) { if (err) *err = "anchor set outside a 4-vertex core"; return -22; }
+  const int nf = (int)lab.size();
+  if (nf > kCliqueMaxFringes) { if (err) *err = "too many fringes for the clique schedule"; return -7; }
+  // set partitions as restricted growth strings
+  std::map<std::vector<int>, long long> T;
+  std::vector<int> blk(nf, 0);
+  std::function<void(int, int)> rec = [&](int i, int nb) {
+    if (i == nf) {
+      std::vector<int> um(nb, 0), sz(nb, 0);
+      for (int q = 0; q < nf; q++) { um[blk[q]] |= lab[q]; sz[blk[q]]++; }
+      long long mu = 1;
+      for (int b = 0; b < nb; b++) {
+        long long fct = 1;
+        for (int k = 2; k < sz[b]; k++) fct *= k;
+        mu *= (sz[b] % 2 == 1) ? fct : -fct;
+      }
+      std::sort(um.begin(), um.end());
+      T[um] += mu;
+      return;
+    }
+    for (int b = 0; b <= nb; b++) {
+      blk[i] = b;
+      rec(i + 1, b == nb ? nb + 1 : nb);
+    }
+  };
+  rec(0, 0);
+  std::vector<std::pair<std::vector<int>, long long>> base;
+  for (auto& kv : T)
+    if (kv.second) base.push_back(kv);
+  std::memset(tab, 0, sizeof(*tab));
+  tab->thr[0] = j.p[1];
+  for (int i = 0; i < 3; i++) tab->thr[1 + i] = j.p[2 + i];
+  int perm[4] = {0, 1, 2, 3};
+  std::map<std::vector<std::pair<std::vector<int>, long long>>, int> cls_of;
+  std::vector<std::vector<std::pair<std::vector<int>, long long>>> cls_terms;
+  auto mapm = [&](int m) {
+    int r = 0;
+    for (int i = 0; i < 4; i++)
+      if (m >> i & 1) r |= 1 << perm[i];
+    return r;
+  };
+  int pi = 0;
+  do {
+    std::vector<std::pair<std::vector<int>, long long>> mt;
+    for (auto& t : base) {
+      std::vector<int> ms;
+      for (int m : t.first) ms.push_back(mapm(m));
+      std::sort(ms.begin(), ms.end());
+      mt.push_back({ms, t.second});
+    }
+    std::sort(mt.begin(), mt.end());
+    auto it = cls_of.find(mt);
+    int c;
+    if (it == cls_of.end()) {
+      c = (int)cls_terms.size();
+      cls_of[mt] = c;
+      cls_terms.push_back(mt);
+      int k = 0;
+      for (auto& s : sets) {
+        if (s.second < 3) continue;
+        if (k < 8) tab->nck_mask[c][k] = (uint8_t)mapm(s.first);
+        k++;
+      }
+    } else {
+      c = it->second;
+    }
+    tab->cls_perms[c] |= 1u << pi;
+    for (int i = 0; i < 4; i++) tab->perm[pi][i] = (uint8_t)perm[i];
+    pi++;
+  } while (std::next_permutation(perm, perm + 4));
+  tab->ncls = (int)cls_terms.size();
+  terms->clear();
+  for (int c = 0; c < tab->ncls; c++) {
+    tab->term_off[c] = (int)terms->size();
+    for (auto& t : cls_terms[c]) {
+      if (t.second > INT32_MAX || t.second < INT32_MIN) { if (err) *err = "clique term coefficient out of range"; return -7; }
+      uint32_t pk = (uint32_t)t.first.size() << 28;
+      for (size_t f = 0; f < t.first.size(); f++) pk |= (uint32_t)t.first[f] << (4 * f);
+      terms->push_back(CliqueTerm{(int32_t)t.second, pk});
+    }
+  }
+  tab->term_off[tab->ncls] = (int)terms->size();
+  tab->nterm = (int)terms->size();
+  if (tab->nterm > kCliqueMaxTerms) { if (err) *err = "too many clique terms"; return -7; }
+  // den = 2^s * odd; the inverse of odd modulo 2^128 by Newton's iteration
+  int s = 0;
+  while (((den >> s) & 1) == 0) s++;
+  const u128 odd = den >> s;
+  u128 inv = odd;
+  for (int it = 0; it < 8; it++) inv = inv * ((u128)2 - odd * inv);
+  tab->den_shift = s;
+  tab->den_inv_lo = (uint64_t)inv;
+  tab->den_inv_hi = (uint64_t)(inv >> 64);
+  // the reference's binomial C(n, k) (FC:189-197) wraps its u64 once k * C(n, k) >= 2^64:
+  // the smallest such n per anchor set of k >= 3 fringes
+  int k = 0;
+  for (auto& st : sets) {
+    if (st.second < 3) continue;
+    if (k < 8) {
+      const int f = st.second;
+      auto wraps = [&](long long n) {
+        u128 r = 1;
+        for (int i = 1; i <= f; i++) {
+          r = r * (u128)(n - f + i) / (u128)i;
+          if (r >> 64) return true;
+        }
+        return ((r * (u128)f) >> 64) != 0;
+      };
+      long long lo = f, hi = (1ll << 31);
+      while (lo < hi) {
+        const long long mid = (lo + hi) / 2;
+        if (wraps(mid)) hi = mid; else lo = mid + 1;
+      }
+      tab->nck_thr[k] = (int32_t)std::min<long long>(lo, INT32_MAX);
+    }
+    k++;
+  }
+  tab->nck_n = std::min(k, 8);
   return 0;
 }
 

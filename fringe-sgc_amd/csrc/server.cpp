@@ -64,7 +64,11 @@ struct fsgc_server {
 namespace {
 
 // default workgroups of one launch of a schedule when it has the machine to itself
-int full_grid(int sched, int cus) { return (sched == fsgc::kPivot || sched == fsgc::kRank) ? cus * 4 : cus * 8; }
+int full_grid(int sched, int cus)
+{
+  if (sched == fsgc::kClique) return cus * 2;
+  return (sched == fsgc::kPivot || sched == fsgc::kRank) ? cus * 4 : cus * 8;
+}
 
 }  // namespace
 

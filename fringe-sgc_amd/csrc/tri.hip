@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
                                                              int need_lt, int tails, const TriItem* __restrict__ items,
                                                              int nitems,
                                                              int* __restrict__ head, unsigned char* __restrict__ gmark,
-                                                             long long mstride, int markcap)
+                                                             long long mstride, int markcap, int* __restrict__ c3out)
 {
   __shared__ unsigned char s_mark[kTriWaves][kMarkCap];
   const int lane = lane_id();
@@ -231,6 +232,8 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
             if (lane == who) c3 = cnt;
           }
         }
+        // the triple count of this triangle at z's entry of T(x, y) (x < y < z by rank)
+        if (c3out && act) c3out[k] = c3;
         if (act) {
 #pragma unroll
           for (int j = 0; j < NJ; j++) {
@@ -589,7 +592,7 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
   hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, s, idx, adj, tsup,  \
                      adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride, \
-                     markcap)
+                     markcap, nullptr)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
     } else if (nj == 2) {
@@ -599,6 +602,72 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
     }
 #undef FSGC_TRI_LAUNCH
   };
+  return 0;
+}
+
+// The triple count of every triangle {a, b, c} (a < b < c by rank) at c's entry of
+// T(a, b): one rank-schedule pass with no jobs that writes the c3 it computes. Built
+// once per graph, for the clique schedule (4 bytes per list entry).
+int device_c3tl_build(DeviceGraph* g, void* stream_v, double* ms, std::string* err)
+{
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (ms) *ms = 0;
+  if (g->c3tl) return 0;
+  hipStream_t stream = (hipStream_t)stream_v;
+  int rc = device_derived_build(g, stream_v, nullptr, err, true);
+  if (rc) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  int32_t* c3 = nullptr;
+  HIP_TRY(hipMalloc((void**)&c3, sizeof(int32_t) * (size_t)std::max<long long>(1, g->tl_total)));
+  {
+    DevScratch sc(stream);
+    std::vector<dev::TriItem> items;
+    for (int x = 0; x < g->n; x++) {
+      const int dx = g->h_idx[x + 1] - g->h_idx[x];
+      if (dx < 2) continue;
+      const int per = std::max(1, 256 * 64 / std::max(64, dx));
+      for (int p = 0; p < dx; p += per) items.push_back(dev::TriItem{x, p, std::min(dx, p + per), 0});
+    }
+    int dev = 0, cus = 256;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = std::max(1, std::min<int>(cus * 4, ((int)items.size() + dev::kTriWaves - 1) / dev::kTriWaves));
+    const long long mstride = ((long long)g->maxdeg + 63) / 64 * 64;
+    JobDesc job;
+    std::memset(&job, 0, sizeof(job));
+    job.smod = 1;
+    unsigned char* gmark = nullptr;
+    JobDesc* d_job = nullptr;
+    Accum* d_acc = nullptr;
+    dev::TriItem* d_items = nullptr;
+    int* head = nullptr;
+    HIP_TRY(sc.get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
+    HIP_TRY(sc.get(&d_job, sizeof(JobDesc)));
+    HIP_TRY(sc.get(&d_acc, sizeof(Accum)));
+    HIP_TRY(sc.get(&head, sizeof(int)));
+    HIP_TRY(sc.get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+    job.out = d_acc;
+    HIP_TRY(hipMemsetAsync(c3, 0, sizeof(int32_t) * (size_t)std::max<long long>(1, g->tl_total), stream));
+    HIP_TRY(hipMemsetAsync(gmark, 0, (size_t)mstride * grid * dev::kTriWaves, stream));
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(Accum), stream));
+    HIP_TRY(hipMemsetAsync(head, 0, sizeof(int), stream));
+    HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(JobDesc), hipMemcpyHostToDevice, stream));
+    if (!items.empty())
+      HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice,
+                             stream));
+    if (!items.empty()) {
+      hipLaunchKernelGGL((dev::k_triangle_rank<1, true>), dim3(grid), dim3(dev::kTriBlock), 0, stream,
+                         (const int*)g->idx, (const int*)g->adj, (const int*)g->tsup, (const int*)g->adjdeg,
+                         (const long long*)g->tl_off, (const int*)g->tl, (const JobDesc*)d_job, 0, 1, 0, 0,
+                         (const dev::TriItem*)d_items, (int)items.size(), head, gmark, mstride, dev::kMarkCap, c3);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
+  g->c3tl = c3;
+  g->derived_bytes += sizeof(int32_t) * (size_t)std::max<long long>(1, g->tl_total);
+  if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 

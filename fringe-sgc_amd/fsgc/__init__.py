@@ -63,7 +63,7 @@ class ServerStats(C.Structure):
                 ("slots", C.c_int32), ("peak_concurrent", C.c_int32)]
 
 
-SCHEDULE_NAMES = ["per-root", "pivot", "rank", "edge-pass"]
+SCHEDULE_NAMES = ["per-root", "pivot", "rank", "edge-pass", "clique"]
 RELABEL_ORDERS = {"desc": 0, "asc": 1}
 
 
@@ -279,7 +279,7 @@ class Graph:
         """Times launch shapes per schedule on every sample-th work item and keeps the
         fastest for this graph (used by later counts for fields left at zero)."""
         buf = b"".join(mos)
-        reps = (TuneReport * 4)()
+        reps = (TuneReport * 5)()
         nr = C.c_int32()
         _check(lib.fsgc_autotune(self._h, buf, len(mos), sample, reps, C.byref(nr)))
         out = []

@@ -132,9 +132,12 @@ int fsgc_count(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], const fsgc_tune* 
 int fsgc_count_noalloc(fsgc_graph* g, const uint8_t mo[FSGC_MO_BYTES], fsgc_result* out);
 
 /* The kernel schedule fsgc_count uses for a motif on a graph of maximum degree maxdeg:
-   0 per-root, 1 pivot, 2 rank, 3 edge pass, -1 none (vertex or edge count, computed on
-   the host). Wedge cores take the per-root schedule when maxdeg >= 2^23, where the
-   pivot kernel's 24-bit 2-hop counters could overflow. Host only: no GPU is touched.
+   0 per-root, 1 pivot, 2 rank, 3 edge pass, 4 clique, -1 none (vertex or edge count,
+   computed on the host). Wedge cores take the per-root schedule when maxdeg >= 2^23,
+   where the pivot kernel's 24-bit 2-hop counters could overflow. 4-clique general cores
+   take the clique schedule when their fringe count stays below 2^128 for every
+   clique of the graph (maxdeg^fringes < 2^128, maxdeg < 2^21, at most 7 fringes), else
+   the per-root schedule. Host only: no GPU is touched.
    No reference counterpart (the reference's dispatch sits inside occurrences,
    fringeCount.cu:2199-2262, and has one kernel per class). motif_class may be NULL. */
 int fsgc_plan_schedule(const uint8_t mo[FSGC_MO_BYTES], int32_t maxdeg, int32_t* schedule, int32_t* motif_class);
@@ -156,14 +159,14 @@ int fsgc_batch_run(fsgc_batch* b, fsgc_result* out, double* graph_ms);
 void fsgc_batch_free(fsgc_batch* b);
 
 /* Autotuner. For every schedule the given motifs use (per-root kernel, pivot, rank,
-   edge pass), times candidate launch shapes (workgroups, roots per work item, hub
+   edge pass, clique), times candidate launch shapes (workgroups, roots per work item, hub
    threshold) on every sample-th work item of the graph (sample <= 1: 16) and keeps the
    fastest for the graph; later counts on the graph use it for fields left at zero.
    Counts never depend on the shape. reports[0..*nreports) describe each schedule
    tuned (up to FSGC_NUM_SCHEDULES). */
-#define FSGC_NUM_SCHEDULES 4
+#define FSGC_NUM_SCHEDULES 5
 typedef struct fsgc_tune_report {
-  int32_t schedule;     /* 0 per-root, 1 pivot, 2 rank, 3 edge pass */
+  int32_t schedule;     /* 0 per-root, 1 pivot, 2 rank, 3 edge pass, 4 clique */
   int32_t candidates;   /* shapes timed */
   fsgc_tune best;       /* the shape kept */
   double best_ms;       /* device time of the sample with it */
