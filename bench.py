@@ -117,7 +117,9 @@ def batch_compare(spec, seed=3, count=64, classes=None, out_path=None, emit=True
     print(f"# graph {spec}: n={n} m={m} maxdeg={maxdeg} relabel={relabel} ({time.perf_counter() - t0:.1f} s)",
           flush=True)
     mos_all = fsgc.random_motifs(seed, count)
-    cls_of = {i: fsgc.motif_class(mo) for i, mo in enumerate(mos_all)}
+    # general cores split by schedule: "clique" (4-clique cores) and "general" (the rest)
+    cls_of = {i: ("clique" if fsgc.plan_schedule(mo, maxdeg) == "clique" else fsgc.motif_class(mo))
+              for i, mo in enumerate(mos_all)}
     keep = [i for i in range(count) if classes is None or cls_of[i] in classes]
     mos = [mos_all[i] for i in keep]
     # the automorphism sums (per motif, always whole) and the per-graph index are built
@@ -344,7 +346,8 @@ def main():
     ap.add_argument("--batch-relabel", default="none", choices=["none", "asc", "desc"],
                     help="--batch: count on the graph renumbered by degree")
     ap.add_argument("--batch-merge", default=None, help="comma-separated plain-only and batched-only --batch-out files")
-    ap.add_argument("--classes", default=None, help="comma-separated motif classes for --batch / --serve")
+    ap.add_argument("--classes", default=None, help="comma-separated motif classes for --batch / --serve "
+                    "(--batch also takes clique: the general cores with a 4-clique core)")
     ap.add_argument("--serve", default=None, help="comma-separated graph specs: served Poisson query stream")
     ap.add_argument("--rate", type=float, default=20.0, help="--serve: offered queries per second")
     ap.add_argument("--queries", type=int, default=64, help="--serve: number of queries")

@@ -40,15 +40,30 @@
 namespace fsgc {
 namespace dev {
 
-constexpr int kCqBlock = 256;
+constexpr int kCqBlock = 128;
 constexpr int kCqWaves = kCqBlock / kWave;
-constexpr int kCqMarkCap = 4096;  // LDS marks per wave (positions of N(x))
+constexpr int kCqMarkCap = 2048;  // LDS marks per wave (positions of N(x))
+constexpr int kCqBuf = 2 * kWave;  // clique records per wave awaiting evaluation
+// a record: N[U] for the masks U = 1..15 (rows 0..14), the degrees (15..18) and the ids
+// (19..22) of the clique's vertices in rank order
+constexpr int kRecRows = 23;
 
 struct CqItem {
   int32_t x;
   int32_t p0;  // positions [p0, p1) of N(x) taken as y
   int32_t p1;
   int32_t pad;
+};
+
+// per wave: one batch of up to 64 entries z of T(x, y) ranked above y, whose lists
+// T(x, z) are streamed as one sequence
+struct CqBatch {
+  long long zb[kWave];   // T(x, z) in tl
+  long long zxb[kWave];  // T(z, x) in tl
+  long long yzb[kWave];  // T(y, z) in tl
+  long long yze[kWave];
+  int off[kWave + 1];    // offsets of the lists in the sequence
+  int z[kWave], dz[kWave], zb2[kWave], txz[kWave], tyz[kWave], c3[kWave], zi[kWave];
 };
 
 __device__ __forceinline__ bool rank_lt(int da, int a, int db, int b) { return da < db || (da == db && a < b); }
@@ -71,9 +86,17 @@ __device__ __forceinline__ long long lbound_ll(const int* __restrict__ a, long l
   return l;
 }
 
-// F of assignment class c: (sum of the class's terms modulo 2^128) / den, exact
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// F of assignment class c for the record in column `slot`: (the class's terms summed
+// modulo 2^128) / den, exact
 __device__ __forceinline__ u128 clique_class_value(const CliqueTab* __restrict__ T, int c,
-                                                   const unsigned (*__restrict__ sN)[kCqBlock], int tid)
+                                                   const unsigned (*__restrict__ rec)[kCqBuf], int slot)
 {
   u128 v = 0;
   const CliqueTerm* __restrict__ terms = T->terms;
@@ -84,57 +107,61 @@ __device__ __forceinline__ u128 clique_class_value(const CliqueTab* __restrict__
     const int nf = (int)(pk >> 28);
     ull a = 1;
     int f = 0;
-    // N < 2^21: three factors fit 64 bits, six fit 128
-    for (; f < nf && f < 3; f++) a *= (ull)sN[(pk >> (4 * f)) & 15][tid];
+    // N < 2^21: three factors fit 64 bits
+    for (; f < nf && f < 3; f++) a *= (ull)rec[((pk >> (4 * f)) & 15) - 1][slot];
     u128 prod = a;
     if (f < nf) {
       ull b = 1;
-      for (; f < nf && f < 6; f++) b *= (ull)sN[(pk >> (4 * f)) & 15][tid];
+      for (; f < nf && f < 6; f++) b *= (ull)rec[((pk >> (4 * f)) & 15) - 1][slot];
       prod = (u128)a * b;
-      for (; f < nf; f++) prod = mul_lo128(prod, (u128)sN[(pk >> (4 * f)) & 15][tid]);
+      for (; f < nf; f++) prod = mul_lo128(prod, (u128)rec[((pk >> (4 * f)) & 15) - 1][slot]);
     }
     v += mul_lo128(prod, (u128)(__int128)term.coef);
   }
   return mul_lo128(v >> T->den_shift, ((u128)T->den_inv_hi << 64) | T->den_inv_lo);
 }
 
-// Every job on one clique whose vertices' degrees and ids are d[0..3], K[0..3] (rank
-// order) and whose N values this lane holds in sN[.][tid].
+// Every job on the records [0, n) of the buffer, one per lane: the assignments that
+// pass the degree filters (FC:617-663, FC:489-564) and the slice, then per class of
+// assignments its count times the number of them.
 template <int NJ>
-__device__ __forceinline__ void clique_jobs(const CliqueTab* __restrict__ tabs, int njobs, int smod, int srem,
-                                            const int d[4], const int K[4], const unsigned (*__restrict__ sN)[kCqBlock],
-                                            int tid, LaneAcc* acc)
+__device__ __forceinline__ void clique_eval(const CliqueTab* __restrict__ tabs, int njobs, int smod, int srem,
+                                            const unsigned (*__restrict__ rec)[kCqBuf], int n, LaneAcc* acc)
 {
+  const int lane = lane_id();
+  const bool on = lane < n;
+  int d0 = 0, d1 = 0, d2 = 0, d3 = 0, k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+  if (on) {
+    d0 = (int)rec[15][lane]; d1 = (int)rec[16][lane]; d2 = (int)rec[17][lane]; d3 = (int)rec[18][lane];
+    k0 = (int)rec[19][lane]; k1 = (int)rec[20][lane]; k2 = (int)rec[21][lane]; k3 = (int)rec[22][lane];
+  }
+  auto deg = [&](int a) { return a == 0 ? d0 : a == 1 ? d1 : a == 2 ? d2 : d3; };
+  auto vid = [&](int a) { return a == 0 ? k0 : a == 1 ? k1 : a == 2 ? k2 : k3; };
 #pragma unroll
   for (int j = 0; j < NJ; j++) {
     if (j >= njobs) continue;
     const CliqueTab* __restrict__ T = tabs + j;
-    // the assignments that pass the degree filters (FC:617-663, FC:489-564) and the slice
     uint32_t pass = 0;
-    for (int i = 0; i < 24; i++) {
-      const int a0 = T->perm[i][0], a1 = T->perm[i][1], a2 = T->perm[i][2], a3 = T->perm[i][3];
-      bool ok = d[a0] >= T->thr[0] && d[a1] >= T->thr[1] && d[a2] >= T->thr[2] && d[a3] >= T->thr[3];
-      if (ok && smod > 1) ok = (K[a0] % smod) == srem;
-      pass |= (uint32_t)ok << i;
+    if (on) {
+      for (int i = 0; i < 24; i++) {
+        const int a0 = T->perm[i][0], a1 = T->perm[i][1], a2 = T->perm[i][2], a3 = T->perm[i][3];
+        bool ok = deg(a0) >= T->thr[0] && deg(a1) >= T->thr[1] && deg(a2) >= T->thr[2] && deg(a3) >= T->thr[3];
+        if (ok && smod > 1) ok = (vid(a0) % smod) == srem;
+        pass |= (uint32_t)ok << i;
+      }
     }
-    if (!pass) continue;
     acc[j].matches += __popc(pass);
     for (int c = 0; c < T->ncls; c++) {
       const int cnt = __popc(pass & T->cls_perms[c]);
-      if (!cnt) continue;
-      const u128 val = clique_class_value(T, c, sN, tid);
-      acc[j].add(mulx(val, (u128)(unsigned)cnt, acc[j].fl));
-      for (int s = 0; s < T->nck_n; s++)
-        if ((int)sN[T->nck_mask[c][s]][tid] >= T->nck_thr[s]) acc[j].fl |= kRefNckInexact;
+      if (!__ballot(cnt > 0)) continue;
+      if (cnt > 0) {
+        const u128 val = clique_class_value(T, c, rec, lane);
+        acc[j].add(mulx(val, (u128)(unsigned)cnt, acc[j].fl));
+        for (int s = 0; s < T->nck_n; s++)
+          if ((int)rec[T->nck_mask[c][s] - 1][lane] >= T->nck_thr[s]) acc[j].fl |= kRefNckInexact;
+      }
     }
   }
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 template <int NJ>
@@ -150,18 +177,20 @@ __global__ __launch_bounds__(kCqBlock) void k_clique_rank(const int* __restrict_
                                                           long long mstride, long long rowcap, int markcap)
 {
   __shared__ int s_mark[kCqWaves][kCqMarkCap];
-  __shared__ unsigned s_N[16][kCqBlock];
+  __shared__ CqBatch s_bat[kCqWaves];
+  __shared__ unsigned s_rec[kCqWaves][kRecRows][kCqBuf];
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
-  const int tid = threadIdx.x;
   const long long wid = (long long)blockIdx.x * kCqWaves + wave;
   int* __restrict__ Mg = gmark + wid * mstride;
   int* __restrict__ Z = gzmap + wid * mstride;
   ull* __restrict__ rows = grows + wid * rowcap;
+  CqBatch& B = s_bat[wave];
+  unsigned (*__restrict__ rec)[kCqBuf] = s_rec[wave];
   LaneAcc acc[NJ];
   const int smod = jobs[0].smod, srem = jobs[0].srem;
+  int nrec = 0;  // records in the buffer (wave-uniform)
   for (int i = lane; i < kCqMarkCap; i += kWave) s_mark[wave][i] = 0;
-  s_N[0][tid] = 0;
   wave_sync();
 
   for (;;) {
@@ -183,8 +212,8 @@ __global__ __launch_bounds__(kCqBlock) void k_clique_rank(const int* __restrict_
       const int yb2 = idx[y];
       // T(y, x): the same common neighbours as T(x, y), as positions in N(y)
       const long long yxb = tl_off[lbound_i(adj, yb2, idx[y + 1], x)];
-      // mark T(x, y) with each entry's index + 1; Z[j] = rank among the entries ranked
-      // above y (the row of z = entry j), or -1
+      // mark T(x, y) with each entry's index + 1; Z[j] = the row of z = entry j (its
+      // rank among the entries ranked above y), or -1
       int nz = 0;
       for (int j0 = 0; j0 < t; j0 += kWave) {
         const int j = j0 + lane;
@@ -202,97 +231,154 @@ __global__ __launch_bounds__(kCqBlock) void k_clique_rank(const int* __restrict_
       if (nz >= 2) {
         const int R = (t + 63) >> 6;
         const bool use_rows = (long long)nz * R <= rowcap;
-        // pass 1: row of z = the common neighbours of x, y, z as bits over T(x, y)
         if (use_rows) {
-          for (int j0 = 0; j0 < t; j0 += kWave) {
-            const int j = j0 + lane;
-            if (j >= t) continue;
-            const int zi = Z[j];
-            if (zi < 0) continue;
-            ull* __restrict__ row = rows + (long long)zi * R;
-            for (int r = 0; r < R; r++) row[r] = 0;
-            const int pz = tl[yb + j];
-            const long long zb = tl_off[xb + pz], ze = tl_off[xb + pz + 1];
-            for (long long q = zb; q < ze; q++) {
-              const int mk = M[tl[q]];
-              if (mk) row[(mk - 1) >> 6] |= 1ull << ((mk - 1) & 63);
-            }
-          }
+          for (long long i = lane; i < (long long)nz * R; i += kWave) rows[i] = 0;
           wave_sync();
         }
-        // pass 2: cliques (x, y, z, w), one lane per z
-        for (int j0 = 0; j0 < t; j0 += kWave) {
-          const int j = j0 + lane;
-          if (j >= t) continue;
-          const int zi = Z[j];
-          if (zi < 0) continue;
-          const int pz = tl[yb + j];
-          const int z = adj[xb + pz], dz = adjdeg[xb + pz];
-          const int txz = tsup[xb + pz];
-          const long long zb = tl_off[xb + pz], ze = tl_off[xb + pz + 1];
-          const int zb2 = idx[z];
-          const long long zxb = tl_off[lbound_i(adj, zb2, idx[z + 1], x)];
-          const int pyz = tl[yxb + j];
-          const int tyz = tsup[yb2 + pyz];
-          const long long yzb = tl_off[yb2 + pyz], yze = tl_off[yb2 + pyz + 1];
-          const int c3xyz = c3tl[yb + j];
-          const ull* __restrict__ rz = rows + (long long)zi * R;
-          for (long long q = zb; q < ze; q++) {
-            const int pw = tl[q];
-            const int mk = M[pw];
-            if (!mk) continue;
-            const int jw = mk - 1;
-            if (Z[jw] < 0) continue;  // w ranked below y
-            const int w = adj[xb + pw], dw = adjdeg[xb + pw];
-            if (!rank_lt(dz, z, dw, w)) continue;
-            // the clique (x, y, z, w)
-            const int txw = tsup[xb + pw];
-            const int tzw = tsup[zb2 + tl[zxb + (q - zb)]];
-            const int pyw = tl[yxb + jw];
-            const int tyw = tsup[yb2 + pyw];
-            const int c3xyw = c3tl[yb + jw];
-            const int c3xzw = c3tl[q];
-            const int c3yzw = c3tl[lbound_ll(tl, yzb, yze, pyw)];
-            int c4 = 0;
-            if (use_rows) {
-              const ull* __restrict__ rw = rows + (long long)Z[jw] * R;
-              for (int r = 0; r < R; r++) c4 += __popcll(rz[r] & rw[r]);
-            } else {
-              const long long wb = tl_off[xb + pw], we = tl_off[xb + pw + 1];
-              for (long long q2 = zb; q2 < ze; q2++) {
-                const int p2 = tl[q2];
-                if (!M[p2]) continue;
-                const long long h = lbound_ll(tl, wb, we, p2);
-                c4 += (h < we && tl[h] == p2) ? 1 : 0;
+        // sweep 0 writes the bit rows (row z: the common neighbours of x, y, z over the
+        // indices of T(x, y)); sweep 1 takes the cliques, whose quadruple counts read
+        // the rows of both z and w
+        for (int sweep = use_rows ? 0 : 1; sweep < 2; sweep++) {
+          for (int j0 = 0; j0 < t; j0 += kWave) {
+            // the batch: entries j0 .. j0 + 63 of T(x, y) ranked above y
+            const int j = j0 + lane;
+            const bool inz = j < t && Z[j] >= 0;
+            int len = 0;
+            if (inz) {
+              const int pz = tl[yb + j];
+              const long long zb = tl_off[xb + pz];
+              len = (int)(tl_off[xb + pz + 1] - zb);
+              B.zb[lane] = zb;
+              B.zi[lane] = Z[j];
+              if (sweep == 1) {
+                const int z = adj[xb + pz];
+                const int zb2 = idx[z];
+                const int pyz = tl[yxb + j];
+                B.z[lane] = z;
+                B.dz[lane] = adjdeg[xb + pz];
+                B.zb2[lane] = zb2;
+                B.txz[lane] = tsup[xb + pz];
+                B.zxb[lane] = tl_off[lbound_i(adj, zb2, idx[z + 1], x)];
+                B.tyz[lane] = tsup[yb2 + pyz];
+                B.yzb[lane] = tl_off[yb2 + pyz];
+                B.yze[lane] = tl_off[yb2 + pyz + 1];
+                B.c3[lane] = c3tl[yb + j];
               }
             }
-            // N[U], U a mask over (x, y, z, w) = bits 0..3
-            s_N[1][tid] = dx - 3;
-            s_N[2][tid] = dy - 3;
-            s_N[4][tid] = dz - 3;
-            s_N[8][tid] = dw - 3;
-            s_N[3][tid] = txy - 2;
-            s_N[5][tid] = txz - 2;
-            s_N[9][tid] = txw - 2;
-            s_N[6][tid] = tyz - 2;
-            s_N[10][tid] = tyw - 2;
-            s_N[12][tid] = tzw - 2;
-            s_N[7][tid] = c3xyz - 1;
-            s_N[11][tid] = c3xyw - 1;
-            s_N[13][tid] = c3xzw - 1;
-            s_N[14][tid] = c3yzw - 1;
-            s_N[15][tid] = c4;
-            const int d[4] = {dx, dy, dz, dw};
-            const int K[4] = {x, y, z, w};
-            clique_jobs<NJ>(tabs, njobs, smod, srem, d, K, s_N, tid, acc);
+            int incl = len;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+              const int v = __shfl_up(incl, o, kWave);
+              if (lane >= o) incl += v;
+            }
+            const int tot = __shfl(incl, kWave - 1, kWave);
+            B.off[lane] = incl - len;
+            if (lane == kWave - 1) B.off[kWave] = tot;
+            wave_sync();
+            // element g of the sequence is element g - off[k] of lane k's list, k the last
+            // lane with off[k] <= g (empty lists share the next one's offset); k only grows
+            int k = 0;
+            {
+              int lo = 0, hi = kWave;
+              const int g0 = min(lane, max(tot - 1, 0));
+              while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (B.off[mid] <= g0) lo = mid; else hi = mid;
+              }
+              k = lo;
+            }
+            for (int c0 = 0; c0 < tot; c0 += kWave) {
+              const int g = c0 + lane;
+              bool found = false;
+              long long q = 0;
+              int jw = 0, p = 0;
+              if (g < tot) {
+                while (B.off[k + 1] <= g) k++;
+                q = B.zb[k] + (g - B.off[k]);
+                p = tl[q];
+                const int mk = M[p];
+                if (mk) {
+                  jw = mk - 1;
+                  if (sweep == 0) {
+                    atomicOr(&rows[(long long)B.zi[k] * R + (jw >> 6)], 1ull << (jw & 63));
+                  } else if (Z[jw] >= 0) {  // w ranked above y
+                    found = rank_lt(B.dz[k], B.z[k], adjdeg[xb + p], adj[xb + p]);
+                  }
+                }
+              }
+              if (sweep == 0) continue;
+              const ull fm = __ballot(found);
+              if (!fm) continue;
+              if (found) {
+                // the clique (x, y, z, w) into the next free record
+                const int slot = nrec + __popcll(fm & ((1ull << lane) - 1));
+                const int w = adj[xb + p], dw = adjdeg[xb + p];
+                const int pyw = tl[yxb + jw];
+                int c4 = 0;
+                if (use_rows) {
+                  const ull* __restrict__ rz = rows + (long long)B.zi[k] * R;
+                  const ull* __restrict__ rw = rows + (long long)Z[jw] * R;
+                  for (int r = 0; r < R; r++) c4 += __popcll(rz[r] & rw[r]);
+                } else {
+                  const long long wb = tl_off[xb + p], we = tl_off[xb + p + 1];
+                  const long long zb = B.zb[k], ze = B.zb[k] + (B.off[k + 1] - B.off[k]);
+                  for (long long q2 = zb; q2 < ze; q2++) {
+                    const int p2 = tl[q2];
+                    if (!M[p2]) continue;
+                    const long long h = lbound_ll(tl, wb, we, p2);
+                    c4 += (h < we && tl[h] == p2) ? 1 : 0;
+                  }
+                }
+                // N[U] = |intersection of N(v), v in U| - (4 - |U|), U over (x, y, z, w)
+                rec[0][slot] = dx - 3;
+                rec[1][slot] = dy - 3;
+                rec[2][slot] = txy - 2;
+                rec[3][slot] = B.dz[k] - 3;
+                rec[4][slot] = B.txz[k] - 2;
+                rec[5][slot] = B.tyz[k] - 2;
+                rec[6][slot] = B.c3[k] - 1;
+                rec[7][slot] = dw - 3;
+                rec[8][slot] = tsup[xb + p] - 2;
+                rec[9][slot] = tsup[yb2 + pyw] - 2;
+                rec[10][slot] = c3tl[yb + jw] - 1;
+                rec[11][slot] = tsup[B.zb2[k] + tl[B.zxb[k] + (q - B.zb[k])]] - 2;
+                rec[12][slot] = c3tl[q] - 1;
+                rec[13][slot] = c3tl[lbound_ll(tl, B.yzb[k], B.yze[k], pyw)] - 1;
+                rec[14][slot] = c4;
+                rec[15][slot] = dx;
+                rec[16][slot] = dy;
+                rec[17][slot] = B.dz[k];
+                rec[18][slot] = dw;
+                rec[19][slot] = x;
+                rec[20][slot] = y;
+                rec[21][slot] = B.z[k];
+                rec[22][slot] = w;
+              }
+              nrec += __popcll(fm);
+              wave_sync();
+              if (nrec >= kWave) {
+                clique_eval<NJ>(tabs, njobs, smod, srem, rec, kWave, acc);
+                wave_sync();
+                // keep the records past the first 64
+                for (int r = 0; r < kRecRows; r++)
+                  if (lane + kWave < nrec) rec[r][lane] = rec[r][lane + kWave];
+                nrec -= kWave;
+                wave_sync();
+              }
+            }
+            wave_sync();  // the batch arrays are rewritten by the next batch
           }
+          // the row bits were set by atomics in L2: the rows' lines in this CU's L1
+          // (read by an earlier pair) are invalidated before sweep 1 reads them
+          if (sweep == 0) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+          wave_sync();
         }
       }
-      wave_sync();
       for (int j = lane; j < t; j += kWave) M[tl[yb + j]] = 0;
       wave_sync();
     }
   }
+  if (nrec > 0) clique_eval<NJ>(tabs, njobs, smod, srem, rec, nrec, acc);
 #pragma unroll
   for (int j = 0; j < NJ; j++)
     if (j < njobs) flush(acc[j], jobs[j].out);
@@ -318,7 +404,7 @@ static int clique_grid(const Tune& t)
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  return t.grid > 0 ? t.grid : cus * 2;
+  return t.grid > 0 ? t.grid : cus * 3;
 }
 
 size_t clique_scratch_bytes(const DeviceGraph& g, const Tune& t)

@@ -66,7 +66,7 @@ namespace {
 // default workgroups of one launch of a schedule when it has the machine to itself
 int full_grid(int sched, int cus)
 {
-  if (sched == fsgc::kClique) return cus * 2;
+  if (sched == fsgc::kClique) return cus * 3;
   return (sched == fsgc::kPivot || sched == fsgc::kRank) ? cus * 4 : cus * 8;
 }
 

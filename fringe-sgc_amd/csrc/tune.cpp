@@ -28,7 +28,7 @@ std::vector<Cand> candidates(int sched, int cus)
     case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 256}; lds = {0, 1024}; break;
     case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; lds = {0, 2048}; break;
     case fsgc::kEdgePass: grids = {cus * 4, cus * 8, cus * 16}; break;
-    case fsgc::kClique: grids = {cus, cus * 2}; lds = {0, 1024}; break;
+    case fsgc::kClique: grids = {cus * 2, cus * 3}; lds = {0, 1024}; break;
     default: grids = {cus * 4, cus * 8, cus * 16}; hubs = {1024, 4096}; break;
   }
   for (int b : blocks)
@@ -84,7 +84,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     fsgc_tune def;
     std::memset(&def, 0, sizeof(def));
     def.block = s == fsgc::kPivot ? 1024 : 256;
-    def.grid = s == fsgc::kPivot ? cus : (s == fsgc::kRank ? cus * 4 : (s == fsgc::kClique ? cus * 2 : cus * 8));
+    def.grid = s == fsgc::kPivot ? cus : (s == fsgc::kRank ? cus * 4 : (s == fsgc::kClique ? cus * 3 : cus * 8));
     def.chunk = d.chunk;
     def.hub_degree = d.hub_degree;
     def.lds_cap = d.lds_cap;
