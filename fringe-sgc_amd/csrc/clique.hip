@@ -141,8 +141,21 @@ __device__ __forceinline__ void clique_eval(const CliqueTab* __restrict__ tabs, 
   for (int j = 0; j < NJ; j++) {
     if (j >= njobs) continue;
     const CliqueTab* __restrict__ T = tabs + j;
+    constexpr uint32_t kAll = (1u << 24) - 1;
     uint32_t pass = 0;
-    if (on) {
+    const int dmin = min(min(d0, d1), min(d2, d3));
+    if (on && dmin >= T->max_thr) {
+      // every vertex meets every position's threshold; a slice keeps the assignments
+      // whose root is in it
+      if (smod <= 1) {
+        pass = kAll;
+      } else {
+        if (k0 % smod == srem) pass |= T->root_perms[0];
+        if (k1 % smod == srem) pass |= T->root_perms[1];
+        if (k2 % smod == srem) pass |= T->root_perms[2];
+        if (k3 % smod == srem) pass |= T->root_perms[3];
+      }
+    } else if (on) {
       for (int i = 0; i < 24; i++) {
         const int a0 = T->perm[i][0], a1 = T->perm[i][1], a2 = T->perm[i][2], a3 = T->perm[i][3];
         bool ok = deg(a0) >= T->thr[0] && deg(a1) >= T->thr[1] && deg(a2) >= T->thr[2] && deg(a3) >= T->thr[3];
@@ -151,14 +164,21 @@ __device__ __forceinline__ void clique_eval(const CliqueTab* __restrict__ tabs, 
       }
     }
     acc[j].matches += __popc(pass);
+    // a clique that passes every assignment takes the merged table of all of them
+    const bool all = T->use_all && pass == kAll;
+    if (__ballot(all)) {
+      if (all) acc[j].add(clique_class_value(T, T->ncls, rec, lane));
+    }
     for (int c = 0; c < T->ncls; c++) {
       const int cnt = __popc(pass & T->cls_perms[c]);
-      if (!__ballot(cnt > 0)) continue;
-      if (cnt > 0) {
-        const u128 val = clique_class_value(T, c, rec, lane);
-        acc[j].add(mulx(val, (u128)(unsigned)cnt, acc[j].fl));
+      if (cnt > 0)
         for (int s = 0; s < T->nck_n; s++)
           if ((int)rec[T->nck_mask[c][s] - 1][lane] >= T->nck_thr[s]) acc[j].fl |= kRefNckInexact;
+      const bool need = cnt > 0 && !all;
+      if (!__ballot(need)) continue;
+      if (need) {
+        const u128 val = clique_class_value(T, c, rec, lane);
+        acc[j].add(mulx(val, (u128)(unsigned)cnt, acc[j].fl));
       }
     }
   }
@@ -434,6 +454,16 @@ int plan_cliques(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, void
     if (!clique_fits(jobs[k], g->maxdeg)) { if (err) *err = "motif not countable on the clique schedule"; return -22; }
     rc = build_clique_tab(jobs[k], &tabs[k], &terms[k], err);
     if (rc) return rc;
+    // the merged table sums 24 counts, each below maxdeg^fringes / den
+    {
+      unsigned __int128 p = 24;
+      bool ok = true;
+      for (int f = job_fringes(jobs[k]); f > 0 && ok; f--) {
+        if (p > (~(unsigned __int128)0) / (unsigned __int128)std::max(1, g->maxdeg)) ok = false;
+        else p *= (unsigned __int128)std::max(1, g->maxdeg);
+      }
+      tabs[k].use_all = ok ? 1 : 0;
+    }
     if (jobs[k].smod != jobs[0].smod || jobs[k].srem != jobs[0].srem) {
       if (err) *err = "clique jobs of one launch take the same root slice";
       return -22;

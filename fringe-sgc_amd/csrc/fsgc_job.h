@@ -42,6 +42,16 @@ struct Accum {
   uint64_t bytes;  // algorithmic global-memory bytes of the launch's schedule (pivot kernel)
 };
 
+// A term of an inclusion-exclusion fringe count (4-clique cores):
+//   F * den = sum_t coef_t * prod_f N[mask_tf],
+// where N[U] is the number of non-core vertices adjacent to every core vertex in U and
+// den = prod over anchor sets of fcnt!. A term packs its factor count in bits 28..31
+// and up to seven 4-bit core-vertex masks below.
+struct CliqueTerm {
+  int32_t coef;
+  uint32_t packed;
+};
+
 struct JobDesc {
   int32_t cls;
   int32_t kind;
@@ -82,18 +92,6 @@ struct JobDesc {
   Accum* out;
 };
 
-// 4-clique core (general core whose four vertices are pairwise adjacent), counted per
-// clique: the fringe count of one ordered embedding is an inclusion-exclusion over
-// the ways the labelled fringes can coincide,
-//   F * den = sum_t coef_t * prod_f N[mask_tf],
-// where N[U] is the number of non-core vertices adjacent to every clique vertex in U
-// and den = prod over anchor sets of fcnt!. A term packs its factor count in bits
-// 28..31 and up to seven 4-bit clique-vertex masks below.
-struct CliqueTerm {
-  int32_t coef;
-  uint32_t packed;
-};
-
 constexpr int kCliqueMaxTerms = 8192;  // terms of all assignment classes of one motif
 constexpr int kCliqueMaxFringes = 7;
 
@@ -102,11 +100,15 @@ constexpr int kCliqueMaxFringes = 7;
 // symmetry of the anchored motif); per class the terms of its first assignment.
 struct CliqueTab {
   int32_t thr[4];          // minimum degree per core position (startdeg, destdeg[0..2])
+  int32_t max_thr;         // the largest of them: a clique of smaller degrees all above it passes every assignment
   int32_t ncls;
   int32_t nterm;
+  int32_t use_all;         // the merged table is exact on this graph (24 * maxdeg^fringes < 2^128)
   uint32_t cls_perms[24];  // per class: bit i set for assignment i
   uint8_t perm[24][4];     // assignment i: core position -> clique vertex (0..3)
-  int32_t term_off[25];    // terms of class c: [term_off[c], term_off[c + 1])
+  uint32_t root_perms[4];  // per clique vertex: the assignments that put it at position 0 (the root)
+  int32_t term_off[26];    // terms of class c: [term_off[c], term_off[c + 1]); class ncls is the
+                           // sum over all 24 assignments (a clique that passes every filter)
   int32_t den_shift;       // den = 2^den_shift * odd
   uint64_t den_inv_lo;     // inverse of odd modulo 2^128
   uint64_t den_inv_hi;

@@ -246,6 +246,99 @@ int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
   return 0;
 }
 
+// ---------------------------------------------------------------- inclusion-exclusion
+
+namespace {
+
+// The fringes as labelled items, item q anchored on the core-vertex mask lab[q]. By
+// Moebius inversion over the set partitions P of the items, the placements of the
+// items on distinct non-core vertices, each adjacent to all of its anchor set, number
+//   sum_P prod_{B in P} (-1)^{|B|-1} (|B|-1)! * N[union of the anchor sets in B];
+// returned as (sorted unions, coefficient) with equal multisets merged.
+std::vector<std::pair<std::vector<int>, long long>> ie_terms(const std::vector<int>& lab)
+{
+  const int nf = (int)lab.size();
+  std::map<std::vector<int>, long long> T;
+  std::vector<int> blk(nf, 0);
+  std::function<void(int, int)> rec = [&](int i, int nb) {
+    if (i == nf) {
+      std::vector<int> um(nb, 0), sz(nb, 0);
+      for (int q = 0; q < nf; q++) { um[blk[q]] |= lab[q]; sz[blk[q]]++; }
+      long long mu = 1;
+      for (int b = 0; b < nb; b++) {
+        long long fct = 1;
+        for (int k = 2; k < sz[b]; k++) fct *= k;
+        mu *= (sz[b] % 2 == 1) ? fct : -fct;
+      }
+      std::sort(um.begin(), um.end());
+      T[um] += mu;
+      return;
+    }
+    for (int b = 0; b <= nb; b++) {
+      blk[i] = b;
+      rec(i + 1, b == nb ? nb + 1 : nb);
+    }
+  };
+  rec(0, 0);
+  std::vector<std::pair<std::vector<int>, long long>> out;
+  for (auto& kv : T)
+    if (kv.second) out.push_back(kv);
+  return out;
+}
+
+uint32_t ie_pack(const std::vector<int>& masks)
+{
+  uint32_t pk = (uint32_t)masks.size() << 28;
+  for (size_t f = 0; f < masks.size(); f++) pk |= (uint32_t)masks[f] << (4 * f);
+  return pk;
+}
+
+// den = 2^shift * odd; the inverse of odd modulo 2^128 by Newton's iteration
+void ie_den(u128 den, int32_t* shift, uint64_t* lo, uint64_t* hi)
+{
+  int s = 0;
+  while (((den >> s) & 1) == 0) s++;
+  const u128 odd = den >> s;
+  u128 inv = odd;
+  for (int it = 0; it < 8; it++) inv = inv * ((u128)2 - odd * inv);
+  *shift = s;
+  *lo = (uint64_t)inv;
+  *hi = (uint64_t)(inv >> 64);
+}
+
+// the reference's binomial C(n, k) (FC:189-197) wraps its u64 once k * C(n, k) >= 2^64:
+// the smallest such n
+int32_t nck_wrap_threshold(int k)
+{
+  auto wraps = [&](long long n) {
+    u128 r = 1;
+    for (int i = 1; i <= k; i++) {
+      r = r * (u128)(n - k + i) / (u128)i;
+      if (r >> 64) return true;
+    }
+    return ((r * (u128)k) >> 64) != 0;
+  };
+  long long lo = k, hi = (1ll << 31);
+  while (lo < hi) {
+    const long long mid = (lo + hi) / 2;
+    if (wraps(mid)) hi = mid; else lo = mid + 1;
+  }
+  return (int32_t)std::min<long long>(lo, INT32_MAX);
+}
+
+bool maxdeg_pow_fits(int32_t maxdeg, int f)
+{
+  if (maxdeg < 0 || maxdeg >= (1 << 21)) return false;
+  u128 p = 1;
+  for (int i = 0; i < f; i++) {
+    if (maxdeg > 0 && p > (~(u128)0) / (u128)(unsigned)maxdeg) return false;
+    p *= (u128)(unsigned)maxdeg;
+  }
+  return true;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------- 4-clique cores
 
 bool clique_core(const JobDesc& j)
@@ -267,12 +360,7 @@ bool clique_fits(const JobDesc& j, int32_t maxdeg)
 {
   if (!clique_core(j)) return false;
   const int f = job_fringes(j);
-  if (f > kCliqueMaxFringes || maxdeg < 0 || maxdeg >= (1 << 21)) return false;
-  u128 p = 1;
-  for (int i = 0; i < f; i++) {
-    if (maxdeg > 0 && p > (~(u128)0) / (u128)(unsigned)maxdeg) return false;
-    p *= (u128)(unsigned)maxdeg;
-  }
+  if (f > kCliqueMaxFringes || !maxdeg_pow_fits(maxdeg, f)) return false;
   CliqueTab tab;
   std::vector<CliqueTerm> terms;
   return build_clique_tab(j, &tab, &terms, nullptr) == 0;
@@ -305,35 +393,11 @@ int build_clique_tab(const JobDesc& j, CliqueTab* tab, std::vector<CliqueTerm>* 
     if (j.fcnt_by_mask[m]) { if (err) *err = "anchor set outside a 4-vertex core"; return -22; }
   const int nf = (int)lab.size();
   if (nf > kCliqueMaxFringes) { if (err) *err = "too many fringes for the clique schedule"; return -7; }
-  // set partitions as restricted growth strings
-  std::map<std::vector<int>, long long> T;
-  std::vector<int> blk(nf, 0);
-  std::function<void(int, int)> rec = [&](int i, int nb) {
-    if (i == nf) {
-      std::vector<int> um(nb, 0), sz(nb, 0);
-      for (int q = 0; q < nf; q++) { um[blk[q]] |= lab[q]; sz[blk[q]]++; }
-      long long mu = 1;
-      for (int b = 0; b < nb; b++) {
-        long long fct = 1;
-        for (int k = 2; k < sz[b]; k++) fct *= k;
-        mu *= (sz[b] % 2 == 1) ? fct : -fct;
-      }
-      std::sort(um.begin(), um.end());
-      T[um] += mu;
-      return;
-    }
-    for (int b = 0; b <= nb; b++) {
-      blk[i] = b;
-      rec(i + 1, b == nb ? nb + 1 : nb);
-    }
-  };
-  rec(0, 0);
-  std::vector<std::pair<std::vector<int>, long long>> base;
-  for (auto& kv : T)
-    if (kv.second) base.push_back(kv);
+  const std::vector<std::pair<std::vector<int>, long long>> base = ie_terms(lab);
   std::memset(tab, 0, sizeof(*tab));
   tab->thr[0] = j.p[1];
   for (int i = 0; i < 3; i++) tab->thr[1 + i] = j.p[2 + i];
+  tab->max_thr = std::max(std::max(tab->thr[0], tab->thr[1]), std::max(tab->thr[2], tab->thr[3]));
   int perm[4] = {0, 1, 2, 3};
   std::map<std::vector<std::pair<std::vector<int>, long long>>, int> cls_of;
   std::vector<std::vector<std::pair<std::vector<int>, long long>>> cls_terms;
@@ -369,54 +433,41 @@ int build_clique_tab(const JobDesc& j, CliqueTab* tab, std::vector<CliqueTerm>* 
       c = it->second;
     }
     tab->cls_perms[c] |= 1u << pi;
+    tab->root_perms[perm[0]] |= 1u << pi;
     for (int i = 0; i < 4; i++) tab->perm[pi][i] = (uint8_t)perm[i];
     pi++;
   } while (std::next_permutation(perm, perm + 4));
   tab->ncls = (int)cls_terms.size();
+  // the sum over all 24 assignments, merged: what a clique whose every assignment passes
+  // the degree filters adds
+  {
+    std::map<std::vector<int>, long long> all;
+    for (int c = 0; c < tab->ncls; c++) {
+      const long long mult = __builtin_popcount(tab->cls_perms[c]);
+      for (auto& t : cls_terms[c]) all[t.first] += mult * t.second;
+    }
+    std::vector<std::pair<std::vector<int>, long long>> v;
+    for (auto& kv : all)
+      if (kv.second) v.push_back(kv);
+    cls_terms.push_back(v);
+  }
   terms->clear();
-  for (int c = 0; c < tab->ncls; c++) {
+  for (int c = 0; c <= tab->ncls; c++) {
     tab->term_off[c] = (int)terms->size();
     for (auto& t : cls_terms[c]) {
       if (t.second > INT32_MAX || t.second < INT32_MIN) { if (err) *err = "clique term coefficient out of range"; return -7; }
-      uint32_t pk = (uint32_t)t.first.size() << 28;
-      for (size_t f = 0; f < t.first.size(); f++) pk |= (uint32_t)t.first[f] << (4 * f);
+      const uint32_t pk = ie_pack(t.first);
       terms->push_back(CliqueTerm{(int32_t)t.second, pk});
     }
   }
-  tab->term_off[tab->ncls] = (int)terms->size();
+  tab->term_off[tab->ncls + 1] = (int)terms->size();
   tab->nterm = (int)terms->size();
   if (tab->nterm > kCliqueMaxTerms) { if (err) *err = "too many clique terms"; return -7; }
-  // den = 2^s * odd; the inverse of odd modulo 2^128 by Newton's iteration
-  int s = 0;
-  while (((den >> s) & 1) == 0) s++;
-  const u128 odd = den >> s;
-  u128 inv = odd;
-  for (int it = 0; it < 8; it++) inv = inv * ((u128)2 - odd * inv);
-  tab->den_shift = s;
-  tab->den_inv_lo = (uint64_t)inv;
-  tab->den_inv_hi = (uint64_t)(inv >> 64);
-  // the reference's binomial C(n, k) (FC:189-197) wraps its u64 once k * C(n, k) >= 2^64:
-  // the smallest such n per anchor set of k >= 3 fringes
+  ie_den(den, &tab->den_shift, &tab->den_inv_lo, &tab->den_inv_hi);
   int k = 0;
   for (auto& st : sets) {
     if (st.second < 3) continue;
-    if (k < 8) {
-      const int f = st.second;
-      auto wraps = [&](long long n) {
-        u128 r = 1;
-        for (int i = 1; i <= f; i++) {
-          r = r * (u128)(n - f + i) / (u128)i;
-          if (r >> 64) return true;
-        }
-        return ((r * (u128)f) >> 64) != 0;
-      };
-      long long lo = f, hi = (1ll << 31);
-      while (lo < hi) {
-        const long long mid = (lo + hi) / 2;
-        if (wraps(mid)) hi = mid; else lo = mid + 1;
-      }
-      tab->nck_thr[k] = (int32_t)std::min<long long>(lo, INT32_MAX);
-    }
+    if (k < 8) tab->nck_thr[k] = nck_wrap_threshold(st.second);
     k++;
   }
   tab->nck_n = std::min(k, 8);
