@@ -44,6 +44,9 @@ WORKLOADS = {
         # counted on the graph renumbered by ascending degree (2.24 s against 3.47 s on
         # the generator's ids and 3.31 s on the reference's descending order)
         "relabel": "asc",
+        # the motif's count is linear in the triple region: the pivot kernel runs
+        # without c3 and k_wedge_lin_centres adds the per-centre terms
+        "linear": True,
     },
     "c1": {
         "spec": "er:4096:32768:1",
@@ -444,6 +447,25 @@ def main():
         "alg_bytes_per_launch": r.alg_bytes,
     }
 
+    if W.get("linear"):
+        # the same count with every wedge evaluated with its own triple count (the
+        # kernel's c3 instantiation): an independent evaluation of the headline number
+        os.environ["FSGC_PIVOT_C3_PER_WEDGE"] = "1"
+        try:
+            pw = gc.count(mo)
+        finally:
+            os.environ.pop("FSGC_PIVOT_C3_PER_WEDGE", None)
+        pw_ach = pw.alg_bytes / (pw.kernel_ms / 1000.0) / 1e9 if pw.alg_bytes else None
+        roofline["launch"] = "k_wedge_pivot without c3 + k_wedge_lin_centres (per-centre triple-region terms)"
+        roofline["per_wedge_c3"] = {
+            "kernel_ms": round(pw.kernel_ms, 3), "alg_bytes_per_launch": pw.alg_bytes,
+            "achieved": round(pw_ach, 2) if pw_ach else None,
+            "frac": round(pw_ach / HBM_PEAK_GBS, 5) if pw_ach else None,
+            "identical": (pw.exact, pw.sg, pw.ref_compat, pw.core_matches, pw.flags) ==
+                         (r.exact, r.sg, r.ref_compat, r.core_matches, r.flags)}
+        print(f"# per-wedge c3: kernel {pw.kernel_ms:.1f} ms, identical {roofline['per_wedge_c3']['identical']}",
+              flush=True)
+
     cpu = None
     if not args.no_cpu:
         # The oracle (the reference's per-root schedule on the host cores) on a
@@ -486,7 +508,8 @@ def main():
             "cores": cores,
             "kind": "port",
             "sample": (f"roots u % {mod} in [{done[-1]}, {done[0]}] ({len(done)} slices, {matches} core matches, "
-                       f"{secs:.2f} s; reference per-root schedule, OpenMP); GPU ({W['kernel']}) results on "
+                       f"{secs:.2f} s; reference per-root schedule, OpenMP); GPU ({W['kernel']}"
+                       f"{', per-wedge c3 on root slices' if W.get('linear') else ''}) results on "
                        f"{checked} of these slices identical: {parity}"),
         }
         if not parity:
@@ -496,11 +519,18 @@ def main():
             # per-unit sizes of the kernel: they must give the kernel's own figure
             t1 = time.perf_counter()
             cni, cnl = gc.csr()
-            units = oracle.pivot_units(cni, cnl, ecap=2048, hub_slots=0, c3=True, threads=cores)
-            model = oracle.pivot_bytes(units, 2 if maxdeg <= 65536 else 4, hub_launch=False)
-            roofline["alg_model"] = {"units": units, "bytes_per_unit": "20 paths_d + 4 paths_hub + 16 marks_d + 4 marks_hub + "
-                                     "12 xvisits + 4 xvisit_bits + 16 centres + 4 centre_bits + (16+s) c3_x + "
-                                     "(8+s) c3_search_bits + s list_elems + 16 walk_d + 12 walk_hub, s = list element "
+            lin = bool(W.get("linear"))
+            # a launch without c3 keeps the hub vertices' 2-hop counters in LDS; the
+            # per-centre pass reads 8 bytes per vertex and per position
+            units = oracle.pivot_units(cni, cnl, ecap=2048, hub_slots=39552 if lin else 0, c3=not lin, threads=cores)
+            model = oracle.pivot_bytes(units, 2 if maxdeg <= 65536 else 4, hub_launch=lin)
+            if lin:
+                units["centre_pass_vertices"], units["centre_pass_positions"] = n, m
+                model += 8 * (n + m)
+            roofline["alg_model"] = {"units": units, "bytes_per_unit": "20 paths_d + 4 paths_hub + 16 marks_d (20 in a "
+                                     "launch without c3) + 4 marks_hub + 12 xvisits + 4 xvisit_bits + 16 centres + "
+                                     "4 centre_bits + (16+s) c3_x + (8+s) c3_search_bits + s list_elems + 16 walk_d + "
+                                     "12 walk_hub + 8 centre_pass_vertices + 8 centre_pass_positions, s = list element "
                                      "bytes", "list_elem_bytes": 2 if maxdeg <= 65536 else 4, "bytes": model,
                                      "equals_kernel": int(r.alg_bytes) == model,
                                      "seconds": round(time.perf_counter() - t1, 2)}

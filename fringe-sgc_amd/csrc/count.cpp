@@ -161,7 +161,8 @@ int group_tier(const JobDesc& j)
 {
   const bool multi = needs_full(j);
   if (j.cls == kEdge) return 0;
-  if (j.cls == kWedge) return j.kind == kWedgeSpecial ? 0 : (multi ? 2 : 1);
+  // wedge jobs linear in the triple region need no c3 pass, like the specialised ones
+  if (j.cls == kWedge) return (j.kind == kWedgeSpecial || j.p[11]) ? 0 : (multi ? 2 : 1);
   if (j.cls == kTriangle) return j.kind != kTriGeneral ? 1 : (multi ? 3 : 2);
   return 0;
 }

@@ -421,6 +421,27 @@ __device__ __forceinline__ u128 fc3_xyz(int fx, int fy, int fz, int x, int y, in
   return fc3_collapsed(fx, fy, fz, x, y, z, xy, xz, yz, xyz, fl);
 }
 
+// A wedge core with f fringes on its centre X and one on each endpoint Y, Z (single-
+// vertex anchor sets) has the fringe count fc3 = V - b0 (A + B) + (b0 - b2) s, where
+//   V = b0 [(A - P)(B - Q) - R + A + B] + b1 [P (B - Q) + Q (A - P)] + b2 P Q,
+// b_c = C(n_X - c, f), n_X = |N(X) minus the core|, A = |N(Y) minus the core|,
+// B = |N(Z) minus the core|, P = |A & N(X)|, Q = |B & N(X)|, R = |A & B| and s the
+// triple region |N(X) & N(Y) & N(Z)| (fc3_small's three terms regrouped by s). The
+// count is linear in s with a coefficient of the centre alone, so a sum over wedges
+// takes the s and A + B terms per centre and V per wedge. Every bracket is a count of
+// pairs (R <= min(A, B)), so V >= 0; each stays below 2^63. With inline u64 binomials
+// (fc3_small_inline_ok) V < 2^126 and no flag can arise.
+__device__ __forceinline__ u128 wedge_lin_v(int f, int nx, int a, int b, int p, int q, int r, uint32_t& fl)
+{
+  const ull t0 = (ull)((long long)(a - p) * (b - q) - r + a + b);
+  const ull t1 = (ull)((long long)p * (b - q) + (long long)q * (a - p));
+  const ull t2 = (ull)((long long)p * q);
+  if (fc3_small_inline_ok(f, nx))
+    return (u128)ubin_small(nx, f) * t0 + (u128)ubin_small(nx - 1, f) * t1 + (u128)ubin_small(nx - 2, f) * t2;
+  return addx(addx(mulx(binom(nx, f, fl), (u128)t0, fl), mulx(binom(nx - 1, f, fl), (u128)t1, fl), fl),
+              mulx(binom(nx - 2, f, fl), (u128)t2, fl), fl);
+}
+
 __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, uint32_t& fl)
 {
   // X = the core vertex with the most fringes (ties: u, then v), Y and Z the others in

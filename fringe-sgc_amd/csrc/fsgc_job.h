@@ -66,6 +66,9 @@ struct JobDesc {
   //   edge:     p[0] f_u, p[1] f_uv, p[2] f_v, p[3] lim1, p[4] lim2
   //   wedge:    p[0] min_u, p[1] lim, p[2] sum1, p[3] sum2, p[4] sum3,
   //             special: p[5] f_v, p[6] f_vw, p[7] f_w, p[8] smin_v, p[9] smin_w, p[10] slim
+  //             general: p[11] = 1 when the count is linear in the triple region (f_u >= 1
+  //             fringes on the centre, one on each endpoint, no multi-vertex anchor set;
+  //             the pivot schedule then evaluates it without c3), p[12] f_u
   //   triangle: p[0] sum1, p[1] sum2, p[2] sum3, p[3] tails
   //   general:  p[0] nodes, p[1] startdeg, p[2..6] destdeg, p[7..11] list, p[12] nonanch, p[13] top
   int32_t p[16];

@@ -97,6 +97,10 @@ struct LaunchPlan {
   int nitems = 0;            // work items of the launch
   // launches the kernel over items [first, first + count) with work-queue head `head`
   std::function<void(void* stream, int first, int count, int* head)> kernel;
+  // optional: launched once after all items (per-centre terms of the pivot schedule)
+  std::function<void(void* stream)> post;
+  // the accumulators nj .. 2 nj - 1 hold negative parts, subtracted from the sums
+  bool neg = false;
   int* head = nullptr;       // work-queue heads (32 slots), zeroed with the accumulators
   std::shared_ptr<DevScratch> scratch;
   LaunchPlan() = default;
@@ -106,8 +110,9 @@ struct LaunchPlan {
   LaunchPlan& operator=(LaunchPlan&& o) noexcept;
   ~LaunchPlan();
 };
-// allocates the accumulators (device and pinned host) of a plan for nj jobs
-int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err);
+// allocates the accumulators (device and pinned host) of a plan for nj jobs, and with
+// neg a negative part per job
+int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err, bool neg = false);
 // zero + kernel (+ timing events when given) + copy of the results; no allocation, no sync
 int launch_enqueue(const LaunchPlan& lp, void* stream, void* ev_begin, void* ev_end, std::string* err);
 // results of a finished run

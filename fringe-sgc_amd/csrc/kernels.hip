@@ -586,8 +586,9 @@ LaunchPlan& LaunchPlan::operator=(LaunchPlan&& o) noexcept
   if (this != &o) {
     if (h_acc) (void)hipHostFree(h_acc);
     nj = o.nj; d_acc = o.d_acc; acc_bytes = o.acc_bytes; h_acc = o.h_acc; index_ms = o.index_ms;
-    nitems = o.nitems; head = o.head;
+    nitems = o.nitems; head = o.head; neg = o.neg;
     kernel = std::move(o.kernel);
+    post = std::move(o.post);
     scratch = std::move(o.scratch);
     o.h_acc = nullptr;
     o.d_acc = nullptr;
@@ -600,16 +601,20 @@ LaunchPlan::~LaunchPlan()
   if (h_acc) (void)hipHostFree(h_acc);
 }
 
-int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err)
+int plan_accumulators(LaunchPlan* lp, int nj, void* stream, std::string* err, bool neg)
 {
   if (!lp->scratch) lp->scratch = std::make_shared<DevScratch>((hipStream_t)stream);
   lp->nj = nj;
-  lp->acc_bytes = sizeof(Accum) * nj + 256;  // the work-queue head lives past the accumulators
+  lp->neg = neg;
+  const int nacc = neg ? 2 * nj : nj;
+  lp->acc_bytes = sizeof(Accum) * nacc + 256;  // the work-queue head lives past the accumulators
   HIP_TRY(lp->scratch->get(&lp->d_acc, lp->acc_bytes));
-  lp->head = (int*)((char*)lp->d_acc + sizeof(Accum) * nj + 128);
-  HIP_TRY(hipHostMalloc((void**)&lp->h_acc, sizeof(Accum) * std::max(1, nj), hipHostMallocDefault));
+  lp->head = (int*)((char*)lp->d_acc + sizeof(Accum) * nacc + 128);
+  HIP_TRY(hipHostMalloc((void**)&lp->h_acc, sizeof(Accum) * std::max(1, nacc), hipHostMallocDefault));
   return 0;
 }
+
+static size_t acc_copy_bytes(const LaunchPlan& lp) { return sizeof(Accum) * (lp.neg ? 2 * lp.nj : lp.nj); }
 
 int launch_enqueue(const LaunchPlan& lp, void* stream_v, void* ev_begin, void* ev_end, std::string* err)
 {
@@ -622,8 +627,12 @@ int launch_enqueue(const LaunchPlan& lp, void* stream_v, void* ev_begin, void* e
     lp.kernel(stream, 0, lp.nitems, lp.head);
     HIP_TRY(hipGetLastError());
   }
+  if (lp.post) {
+    lp.post(stream);
+    HIP_TRY(hipGetLastError());
+  }
   if (ev_end) HIP_TRY(hipEventRecord((hipEvent_t)ev_end, stream));
-  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, sizeof(Accum) * lp.nj, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, acc_copy_bytes(lp), hipMemcpyDeviceToHost, stream));
   return 0;
 }
 
@@ -636,6 +645,14 @@ void launch_collect(const LaunchPlan& lp, std::vector<JobResult>* out)
     r.matches = lp.h_acc[k].matches;
     r.flags = lp.h_acc[k].flags;
     r.bytes = lp.h_acc[k].bytes;
+    if (lp.neg) {
+      const Accum& a = lp.h_acc[lp.nj + k];
+      const unsigned __int128 ng = ((unsigned __int128)a.hi << 64) | a.lo;
+      r.flags |= a.flags;
+      // the negative part never exceeds the sum unless a part wrapped at 2^128
+      if (r.sum < ng) r.flags |= kOvf128;
+      r.sum -= ng;
+    }
     r.index_ms = (k == 0) ? lp.index_ms : 0;
   }
 }
@@ -675,6 +692,10 @@ int launch_run(const LaunchPlan& lp, void* stream_v, std::vector<JobResult>* out
     HIP_TRY(hipEventRecord(ev.a, stream));
     lp.kernel(stream, first, last - first, lp.head + s);
     HIP_TRY(hipGetLastError());
+    if (s == nseg - 1 && lp.post) {
+      lp.post(stream);
+      HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(ev.b, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     float ms = 0;
@@ -683,7 +704,7 @@ int launch_run(const LaunchPlan& lp, void* stream_v, std::vector<JobResult>* out
     const char* lg = getenv("FSGC_LOG");
     if (lg && *lg && *lg != '0') fprintf(stderr, "[fsgc]   segment %d/%d: %d items, %.1f ms\n", s + 1, nseg, last - first, ms);
   }
-  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, sizeof(Accum) * lp.nj, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(lp.h_acc, lp.d_acc, acc_copy_bytes(lp), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   if (kernel_ms) *kernel_ms = total;
   launch_collect(lp, out);

@@ -272,18 +272,23 @@ __device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t
   return fc3_singletons(fr, venn, fl);
 }
 
+__device__ __forceinline__ bool eval_wedge_special(const JobDesc* __restrict__ J, int du, int dv, int dw, int cvw,
+                                                   bool tri, u128& val, uint32_t& fl)
+{
+  const int* p = J->p;
+  if (du < 2 || dv < p[1]) return false;
+  const int thres = max(p[10], p[8] + p[9] - dv);
+  if (dw < thres) return false;
+  val = pair_formula(dv, dw, cvw - 1, tri ? 2 : 1, p[5], p[6], p[7], fl);
+  return true;
+}
+
 template <bool kFull>
 __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u, int w, int du, int dv, int dw,
                                            int tuv, int tuw, int cvw, int c3, bool tri, u128& val, uint32_t& fl)
 {
   const int* p = J->p;
-  if (J->kind == kWedgeSpecial) {
-    if (du < 2 || dv < p[1]) return false;
-    const int thres = max(p[10], p[8] + p[9] - dv);
-    if (dw < thres) return false;
-    val = pair_formula(dv, dw, cvw - 1, tri ? 2 : 1, p[5], p[6], p[7], fl);
-    return true;
-  }
+  if (J->kind == kWedgeSpecial) return eval_wedge_special(J, du, dv, dw, cvw, tri, val, fl);
   const int min_u = p[0], lim = p[1], sum1 = p[2], sum2 = p[3], sum3 = p[4];
   if (du < sum1 || dv < max(sum1, sum2 - du)) return false;
   const int thres2 = max(max(sum1, sum3 - du - dv), max(sum2 - du, sum2 - dv));
@@ -318,6 +323,109 @@ __device__ __forceinline__ bool eval_wedge(const JobDesc* __restrict__ J, int u,
   }
   val = s;
   return true;
+}
+
+// A wedge job whose count is linear in the triple region (JobDesc p[11]: f_u fringes on
+// the centre, one on each endpoint), without c3: the value is V (wedge_lin_v) of the
+// wedge, or of the triangle's three rotations, each with its own centre; the per-centre
+// terms -b0 (A + B) + (b0 - b2) s come from k_wedge_lin_centres over every pair of N(u).
+// The value is taken for every pair, also those the reference's filters drop: such a
+// pair has fc3 = 0, so its V cancels its share of the per-centre terms. The filters
+// (the same as eval_wedge's) decide the core-match count and which flags are kept.
+// A triangle is taken once, at the rotation whose centre u is its smallest vertex.
+// the three rotations of a triangle, each vertex the centre once (out of line: the
+// triangles are a small share of the wedges, and inline they cost the walk registers)
+__device__ __noinline__ u128 wedge_lin_tri(int f, int du, int dv, int dw, int tuv, int tuw, int cvw, uint32_t& fl)
+{
+  return addx(addx(wedge_lin_v(f, du - 2, dw - 2, dv - 2, tuw - 1, tuv - 1, cvw - 1, fl),
+                   wedge_lin_v(f, dv - 2, du - 2, dw - 2, tuv - 1, cvw - 1, tuw - 1, fl), fl),
+              wedge_lin_v(f, dw - 2, du - 2, dv - 2, tuw - 1, cvw - 1, tuv - 1, fl), fl);
+}
+
+__device__ __forceinline__ bool eval_wedge_lin(const JobDesc* __restrict__ J, int u, int w, int du, int dv, int dw,
+                                               int tuv, int tuw, int cvw, bool tri, u128& val, bool& match,
+                                               uint32_t& fl)
+{
+  const int* p = J->p;
+  if (tri && !(u < w)) return false;
+  const int min_u = p[0], lim = p[1], sum1 = p[2], sum2 = p[3], sum3 = p[4], f = p[12];
+  bool pass = du >= sum1 && dv >= max(sum1, sum2 - du) &&
+              dw >= max(max(sum1, sum3 - du - dv), max(sum2 - du, sum2 - dv));
+  if (pass && !tri) pass = du >= min_u && dv >= lim && dw >= max(lim, sum2 - dv);
+  uint32_t lf = 0;
+  if (!tri) val = wedge_lin_v(f, du - 2, dw - 1, dv - 1, tuw, tuv, cvw - 1, lf);
+  else val = wedge_lin_tri(f, du, dv, dw, tuv, tuw, cvw, lf);
+  fl |= pass ? lf : (lf & kOvf128);
+  match = pass;
+  return true;
+}
+
+// The per-centre terms of the jobs evaluated by eval_wedge_lin: for every centre u (in
+// the root slice) with b_c = C(d_u - 2 - c, f), (b0 - b2) S_u goes to the job's sum and
+// b0 K_u to its negative part neg[j], where S_u = sum_{x in N(u)} C(t_ux, 2) counts the
+// pairs of N(u) inside one N(x) (the sum of s over the pairs of N(u)) and
+// K_u = (d_u - 1) sum_x d_x - d_u (d_u - 1) - sum_x t_ux is the sum of A + B over them.
+// One wave per vertex; 8 bytes per vertex (its index pair) and per position (support,
+// neighbour degree) join the launch's algorithmic bytes.
+template <int NJ>
+__global__ __launch_bounds__(256) void k_wedge_lin_centres(const int* __restrict__ idx, const int* __restrict__ tsup,
+                                                           const int* __restrict__ adjdeg, int n,
+                                                           const JobDesc* __restrict__ jobs, int njobs,
+                                                           Accum* __restrict__ neg)
+{
+  const int lane = lane_id();
+  const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int nw = (int)(gridDim.x * blockDim.x / kWave);
+  const int smod = jobs[0].smod, srem = jobs[0].srem;
+  LaneAcc pos[NJ], ng[NJ];
+  ull bytes = 0;
+  for (int u = gw; u < n; u += nw) {
+    if (smod > 1 && u % smod != srem) continue;
+    const int ub = idx[u], du = idx[u + 1] - ub;
+    if (lane == 0) bytes += 8 + 8 * (ull)du;
+    if (du < 2) continue;
+    ull s_lo = 0, s_hi = 0, sd = 0, st = 0;
+    for (int k = lane; k < du; k += kWave) {
+      const ull t = (ull)tsup[ub + k];
+      const ull c = (t * (t - 1)) >> 1;  // t < 2^31
+      s_lo += c;
+      s_hi += s_lo < c;
+      sd += (ull)adjdeg[ub + k];
+      st += t;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const ull olo = __shfl_xor(s_lo, o, kWave), ohi = __shfl_xor(s_hi, o, kWave);
+      const ull nlo = s_lo + olo;
+      s_hi += ohi + (nlo < s_lo);
+      s_lo = nlo;
+      sd += __shfl_xor(sd, o, kWave);
+      st += __shfl_xor(st, o, kWave);
+    }
+    if (lane != 0) continue;
+    const u128 S = ((u128)s_hi << 64) | s_lo;
+    const u128 K = (u128)(ull)(du - 1) * sd - (u128)((ull)du * (ull)(du - 1)) - st;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      if (j < njobs && jobs[j].p[11]) {
+        uint32_t fb = 0;  // the wedges of this centre that pass the filters set the binomials' flags
+        const int f = jobs[j].p[12];
+        const u128 b0 = binom(du - 2, f, fb), b2 = binom(du - 4, f, fb);
+        pos[j].add(mulx(b0 - b2, S, pos[j].fl));
+        ng[j].add(mulx(b0, K, ng[j].fl));
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    if (j < njobs) {
+      flush(pos[j], jobs[j].out);
+      flush(ng[j], neg + j);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bytes += __shfl_xor(bytes, o, kWave);
+  if (lane == 0 && bytes) atomicAdd((ull*)&jobs[0].out->bytes, bytes);
 }
 
 // TL: element type of the common-neighbour lists the kernel reads (positions in N(u):
@@ -607,9 +715,22 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           for (int j = 0; j < NJ; j++) {
             if (j < njobs) {
               u128 val;
-              if (eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl)) {
+              bool match = true;
+              bool take;
+              // the 16-wave c3 shape never runs a linear job (a launch with c3 evaluates
+              // them with their c3); the others may run launches without c3
+              if (!(kPivWaves == 16 && kHub == 0) && jobs[j].p[11]) {
+                take = eval_wedge_lin(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, tri, val, match, acc[j].fl);
+              } else if constexpr (kHub > 0) {
+                // the hub shape runs launches without c3: besides the linear jobs only
+                // specialised ones (any other general job makes the launch take c3)
+                take = eval_wedge_special(jobs + j, du, dv, dw, cvw, tri, val, acc[j].fl);
+              } else {
+                take = eval_wedge<kFull>(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, c3, tri, val, acc[j].fl);
+              }
+              if (take) {
                 acc[j].add(val);
-                acc[j].matches++;
+                acc[j].matches += match;
               }
             }
           }
@@ -949,10 +1070,24 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   }
   int rc = device_derived_build(g, stream_v, &lp->index_ms, err, true);
   if (rc) return rc;
-  bool need_c3 = false;
+  // FSGC_PIVOT_C3_PER_WEDGE=1 in the environment (read per launch plan) evaluates the
+  // jobs that are linear in the triple region with their per-wedge c3 like the others
+  // (timing comparisons and tests; the counts are the same)
+  // A root slice evaluates them per wedge too: the reference gives all three rotations
+  // of a triangle to its smallest vertex, while the per-centre terms give each rotation
+  // to its own centre, so only whole counts agree.
+  const char* pw = getenv("FSGC_PIVOT_C3_PER_WEDGE");
+  const bool per_wedge_c3 = (pw && *pw && *pw != '0') || (nj > 0 && jobs[0].smod > 1);
+  bool need_c3 = false, lin = false;
   for (auto& j : jobs) {
     j.idx = g->idx; j.adj = g->adj; j.n = g->n; j.m = (int32_t)g->m;
-    need_c3 = need_c3 || (j.kind == kWedgeGeneral);
+    if (j.kind != kWedgeGeneral || per_wedge_c3) j.p[11] = 0;
+    need_c3 = need_c3 || (j.kind == kWedgeGeneral && !j.p[11]);
+  }
+  // a launch that computes c3 anyway evaluates every job with it
+  for (auto& j : jobs) {
+    if (need_c3) j.p[11] = 0;
+    lin = lin || j.p[11];
   }
   // work items: pivots split into centre ranges of bounded walk cost. Every item of a
   // split pivot repeats its 2-hop scatter (phase A), so a pivot is split only when its
@@ -983,7 +1118,8 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
 
   const int grid = std::max(1, std::min<int>(pivot_grid(t), (int)sorted.size()));
   const long long dstride = ((long long)g->n + 63) / 64 * 64;
-  rc = plan_accumulators(lp, nj, stream, err);
+  // jobs linear in the triple region take a negative part (nj more accumulators)
+  rc = plan_accumulators(lp, nj, stream, err, lin);
   if (rc) return rc;
   DevScratch& sc = *lp->scratch;
   unsigned* dbuf = nullptr;
@@ -1029,6 +1165,24 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const char* l32 = getenv("FSGC_PIVOT_LISTS32");
   const bool lists32 = l32 && *l32 && *l32 != '0';
   lp->nitems = nitems;
+  if (lin) {
+    const int n = g->n;
+    Accum* neg = lp->d_acc + nj;
+    int dev = 0, cus = 256;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int cgrid = std::max(1, std::min(cus * 8, (n + 3) / 4));
+    lp->post = [=](void* sv) {
+      hipStream_t s = (hipStream_t)sv;
+      if (nj == 1)
+        hipLaunchKernelGGL(dev::k_wedge_lin_centres<1>, dim3(cgrid), dim3(256), 0, s, idx, tsup, adjdeg, n, d_jobs, nj, neg);
+      else if (nj == 2)
+        hipLaunchKernelGGL(dev::k_wedge_lin_centres<2>, dim3(cgrid), dim3(256), 0, s, idx, tsup, adjdeg, n, d_jobs, nj, neg);
+      else
+        hipLaunchKernelGGL(dev::k_wedge_lin_centres<dev::kMaxPivotJobs>, dim3(cgrid), dim3(256), 0, s, idx, tsup, adjdeg,
+                           n, d_jobs, nj, neg);
+    };
+  }
   lp->kernel = [=](void* sv, int first, int count, int* head) {
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB, TL, TLP)                                                          \

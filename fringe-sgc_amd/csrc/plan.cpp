@@ -191,6 +191,10 @@ int make_job(const MatchingOrder& mo, JobDesc* j, std::string* err)
       } else {
         j->kind = kWedgeGeneral;
         j->rot = wedge_rotations(f, j->fr);
+        if (f[iuvw] == 0 && f[iuv] == 0 && f[iuw] == 0 && f[ivw] == 0 && f[iv] == 1 && f[iw] == 1) {
+          j->p[11] = 1;
+          j->p[12] = f[iu];
+        }
       }
       break;
     }
