@@ -542,16 +542,27 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     // With hub slots, adjhub[e] names the LDS counter of a hub (>= 0) or ~y otherwise.
     // Waves take the x one at a time; an x with more than kAChunk positions past w is
     // deferred (phase A2) and its chunks shared by all waves.
+    // Four list loads per lane are issued before their (fire-and-forget) atomics.
     auto scatter2 = [&](int k0, int k1) {
-      for (int k = k0 + lane; k < k1; k += kWave) {
-        const int h = kHub > 0 ? adjhub[k] : ~adj[k];
-        if (kHub > 0 && h >= 0) {
-          atomicAdd(&s_hub[h], 2u);
-          bytes += 4;
-        } else {
-          atomicMax(&D[~h], e0);
-          atomicAdd(&D[~h], 2u);
-          bytes += 20;
+      for (int b = k0; b < k1; b += 4 * kWave) {
+        int hv[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int k = b + r * kWave + lane;
+          hv[r] = k < k1 ? (kHub > 0 ? adjhub[k] : ~adj[k]) : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (b + r * kWave + lane >= k1) continue;
+          const int h = hv[r];
+          if (kHub > 0 && h >= 0) {
+            atomicAdd(&s_hub[h], 2u);
+            bytes += 4;
+          } else {
+            atomicMax(&D[~h], e0);
+            atomicAdd(&D[~h], 2u);
+            bytes += 20;
+          }
         }
       }
     };
@@ -693,20 +704,51 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      // walk the window
+      // walk the window, software-pipelined: the loads of a position run two iterations
+      // (128 positions) ahead of its evaluation and its 2-hop counter load one ahead, so
+      // a wave's random D reads overlap the evaluation instead of stalling every step
+      int h1 = 0, t1 = 0, d1 = 0;  // position + 64: loaded
+      int t0 = 0, d0 = 0;          // position: loaded, counter loaded
+      unsigned r0 = 0u;
+      bool g0 = false;             // its counter is in D (not an LDS hub slot)
+      {
+        const int q = q_lo + lane;
+        int h0 = 0;
+        if (q < q_hi) {
+          h0 = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
+          t0 = tsup[ub + q];
+          d0 = adjdeg[ub + q];
+          g0 = !(kHub > 0 && h0 >= 0);
+          r0 = g0 ? D[~h0] : s_hub[kHub > 0 ? h0 : 0];
+        }
+        if (q + kWave < q_hi) {
+          h1 = kHub > 0 ? adjhub[ub + q + kWave] : ~adj[ub + q + kWave];
+          t1 = tsup[ub + q + kWave];
+          d1 = adjdeg[ub + q + kWave];
+        }
+      }
       for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
         const int q = q0 + lane;
+        // the counter of position q + 64, then the loads of q + 128
+        unsigned r1 = 0u;
+        bool g1 = false;
+        if (q + kWave < q_hi) {
+          g1 = !(kHub > 0 && h1 >= 0);
+          r1 = g1 ? D[~h1] : s_hub[kHub > 0 ? h1 : 0];
+        }
+        int h2 = 0, t2 = 0, d2 = 0;
+        if (q + 2 * kWave < q_hi) {
+          h2 = kHub > 0 ? adjhub[ub + q + 2 * kWave] : ~adj[ub + q + 2 * kWave];
+          t2 = tsup[ub + q + 2 * kWave];
+          d2 = adjdeg[ub + q + 2 * kWave];
+        }
         if (q < q_hi) {
-          const int h = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
-          unsigned dv_;
-          if (kHub > 0 && h >= 0) {
-            dv_ = s_hub[h];
-          } else {
-            const unsigned draw = D[~h];
-            dv_ = (draw >> 24) == epoch ? (draw & 0xffffffu) : 0u;
+          unsigned dv_ = r0;
+          if (g0) {
+            dv_ = (r0 >> 24) == epoch ? (r0 & 0xffffffu) : 0u;
             bytes += 4;
           }
-          const int tuv = tsup[ub + q], dv = adjdeg[ub + q];
+          const int tuv = t0, dv = d0;
           const bool tri = dv_ & 1u;
           const int cvw = (int)(dv_ >> 1);
           const int c3 = c3on ? (int)E[q - q_lo] : 0;
@@ -735,6 +777,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
             }
           }
         }
+        r0 = r1; g0 = g1; t0 = t1; d0 = d1;
+        h1 = h2; t1 = t2; d1 = d2;
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1017,12 +1061,18 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
   return 0;
 }
 
-// waves per workgroup of the pivot launch: Tune::block of 256 or 512 threads selects the
-// 4- and 8-wave shapes; the default is the 16-wave one (1024 threads, four waves per
-// SIMD against two for the 64 KB shapes: 6.4 s against 8.0 s on R-MAT s20)
-static int pivot_waves(const Tune& t)
+// waves per workgroup of the pivot launch: Tune::block of 256, 512, 768 or 1024 threads
+// selects the 4-, 8-, 12- and 16-wave shapes. Launches with c3 default to 16 waves (four
+// per SIMD against two for the 64 KB c3 shapes: 6.4 s against 8.0 s on R-MAT s20), and
+// 12 waves take that shape too. Launches without c3 come in 16-, 12- and 8-wave hub
+// shapes (at most 128, 168 and 256 VGPRs per lane) and default to 12 (C2: 690 ms
+// against 751 ms for 16 waves, whose walk spills, and 1066 ms for 8 before the walk
+// was pipelined).
+static int pivot_waves(const Tune& t, bool c3)
 {
-  if (t.block <= 0 || t.block >= 1024) return 16;
+  if (t.block <= 0) return c3 ? 16 : 12;
+  if (t.block >= 1024) return 16;
+  if (t.block >= 768) return 12;
   return t.block >= 512 ? 8 : 4;
 }
 
@@ -1036,7 +1086,7 @@ static int pivot_grid(const Tune& t)
   // as many workgroups as fit at once: one per CU for the 128 KB shapes, two with 64 KB
   // (the queue is dynamic; 256, 512 and 768 workgroups of 16 waves time the same on
   // R-MAT s20, and each holds a D array of n counters)
-  return t.grid > 0 ? t.grid : (pivot_waves(t) > 4 ? cus : cus * 2);
+  return t.grid > 0 ? t.grid : (pivot_waves(t, true) > 4 ? cus : cus * 2);
 }
 
 size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
@@ -1126,7 +1176,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   JobDesc* d_jobs = nullptr;
   dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
-  const int waves = pivot_waves(t);
+  const int waves = pivot_waves(t, need_c3);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
   HIP_TRY(sc.get(&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
@@ -1194,12 +1244,15 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
     if (tl16 && !lists32) FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, uint16_t, tl16); \
     else FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, 0, int, tl);                 \
   } while (0)
-#define FSGC_PIVOT_SHAPE(NJ, FULL)                                                     \
-  do {                                                                                 \
-    if (waves == 16 && c3) FSGC_PIVOT_C3(NJ, FULL, 16, 2048);                           \
-    else if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, FULL, 16, 1, dev::kHubSlots, int, tl); \
-    else if (waves == 8) FSGC_PIVOT_C3(NJ, FULL, 8, 4096);                              \
-    else FSGC_PIVOT_C3(NJ, FULL, 4, 4096);                                              \
+// hub shapes evaluate only the specialised and linear jobs, whatever kFull says
+#define FSGC_PIVOT_SHAPE(NJ, FULL)                                                      \
+  do {                                                                                  \
+    if (waves >= 12 && c3) FSGC_PIVOT_C3(NJ, FULL, 16, 2048);                            \
+    else if (waves == 16) FSGC_PIVOT_LAUNCH(NJ, false, 16, 1, dev::kHubSlots, int, tl); \
+    else if (waves == 12) FSGC_PIVOT_LAUNCH(NJ, false, 12, 1, dev::kHubSlots, int, tl); \
+    else if (waves == 8 && !c3) FSGC_PIVOT_LAUNCH(NJ, false, 8, 1, dev::kHubSlots, int, tl); \
+    else if (waves == 8) FSGC_PIVOT_C3(NJ, FULL, 8, 4096);                               \
+    else FSGC_PIVOT_C3(NJ, FULL, 4, 4096);                                               \
   } while (0)
     if (nj == 1) {
       if (full) FSGC_PIVOT_SHAPE(1, true); else FSGC_PIVOT_SHAPE(1, false);

@@ -25,7 +25,7 @@ std::vector<Cand> candidates(int sched, int cus)
   switch (sched) {
     // the LDS cap shortens the pivot kernel's c3 windows (more windows, shorter LDS
     // clears) and sends more of the rank kernel's vertices to global marks
-    case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 256}; lds = {0, 1024}; break;
+    case fsgc::kPivot: grids = {cus, cus * 2, cus * 4}; blocks = {1024, 768, 256}; lds = {0, 1024}; break;
     case fsgc::kRank: grids = {cus * 2, cus * 4, cus * 8}; lds = {0, 2048}; break;
     case fsgc::kEdgePass: grids = {cus * 4, cus * 8, cus * 16}; break;
     case fsgc::kClique: grids = {cus * 2, cus * 3}; lds = {0, 1024}; break;
@@ -83,7 +83,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     const fsgc::Tune d;
     fsgc_tune def;
     std::memset(&def, 0, sizeof(def));
-    def.block = s == fsgc::kPivot ? 1024 : 256;
+    def.block = s == fsgc::kPivot ? 0 : 256;  // the pivot kernel's default depends on the launch (c3 or not)
     def.grid = s == fsgc::kPivot ? cus : (s == fsgc::kRank ? cus * 4 : (s == fsgc::kClique ? cus * 3 : cus * 8));
     def.chunk = d.chunk;
     def.hub_degree = d.hub_degree;
