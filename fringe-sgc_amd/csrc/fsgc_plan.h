@@ -1,6 +1,7 @@
 // Motif planning (host) and the device driver interface.
 #pragma once
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -67,6 +68,8 @@ struct DeviceGraph {
   long long tl_total = 0;
   long long* piv_cost = nullptr;         // n: sum of neighbour degrees
   std::vector<long long> h_piv_cost;
+  // pivot-schedule work items on the device, per (chunk << 32 | sample): pointer, count
+  std::map<long long, std::pair<void*, int>> piv_items;
   size_t derived_bytes = 0;              // device bytes of the derived arrays
 };
 

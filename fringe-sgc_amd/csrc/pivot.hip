@@ -922,6 +922,8 @@ void device_derived_free(DeviceGraph* g)
   if (g->piv_cost) (void)hipFree(g->piv_cost);
   if (g->c3tl) (void)hipFree(g->c3tl);
   g->c3tl = nullptr;
+  for (auto& kv : g->piv_items) (void)hipFree(kv.second.first);
+  g->piv_items.clear();
   g->tsup = nullptr;
   g->adjdeg = nullptr;
   g->adjhub = nullptr;
@@ -1093,7 +1095,7 @@ size_t pivot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
   const size_t grid = (size_t)pivot_grid(t);
   const size_t dstride = ((size_t)g.n + 63) / 64 * 64;
-  return sizeof(unsigned) * grid * dstride + sizeof(dev::PivotItem) * ((size_t)g.n + 1024);
+  return sizeof(unsigned) * grid * dstride;  // the work items stay with the graph
 }
 
 // The cost above which a pivot's centres are split over several work items: 1/4096 of
@@ -1143,30 +1145,50 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   // split pivot repeats its 2-hop scatter (phase A), so a pivot is split only when its
   // cost exceeds an eighth of one workgroup's share of the whole launch; heaviest-first
   // order then balances the rest.
-  long long total_cost = 0;
-  for (int w = 0; w < g->n; w++) total_cost += g->h_piv_cost[w];
-  const long long budget = pivot_split_budget(total_cost, t.chunk);
-  std::vector<dev::PivotItem> items;
-  std::vector<long long> icost;
-  for (int w = 0; w < g->n; w++) {
-    const int dw = g->h_idx[w + 1] - g->h_idx[w];
-    if (dw == 0) continue;
-    const long long c = g->h_piv_cost[w];
-    const long long k = std::max<long long>(1, std::min<long long>(dw, c / std::max(budget, c / 8 + 1)));
-    for (long long s = 0; s < k; s++) {
-      const int p0 = (int)(dw * s / k), p1 = (int)(dw * (s + 1) / k);
-      if (p1 > p0) { items.push_back(dev::PivotItem{w, p0, p1, 0}); icost.push_back(c / k + c / 2); }
+  // The items depend on the graph, the chunk and the sample only: they are built once per
+  // graph and key and kept on the device until the graph is freed (a launch still running
+  // on another stream may read them).
+  const dev::PivotItem* d_items = nullptr;
+  int nitems = 0;
+  {
+    static std::mutex items_mu;
+    std::lock_guard<std::mutex> lk(items_mu);
+    const long long key = ((long long)t.chunk << 32) | (unsigned)std::max(1, t.sample);
+    auto it = g->piv_items.find(key);
+    if (it == g->piv_items.end()) {
+      long long total_cost = 0;
+      for (int w = 0; w < g->n; w++) total_cost += g->h_piv_cost[w];
+      const long long budget = pivot_split_budget(total_cost, t.chunk);
+      std::vector<dev::PivotItem> items;
+      std::vector<long long> icost;
+      for (int w = 0; w < g->n; w++) {
+        const int dw = g->h_idx[w + 1] - g->h_idx[w];
+        if (dw == 0) continue;
+        const long long c = g->h_piv_cost[w];
+        const long long k = std::max<long long>(1, std::min<long long>(dw, c / std::max(budget, c / 8 + 1)));
+        for (long long s = 0; s < k; s++) {
+          const int p0 = (int)(dw * s / k), p1 = (int)(dw * (s + 1) / k);
+          if (p1 > p0) { items.push_back(dev::PivotItem{w, p0, p1, 0}); icost.push_back(c / k + c / 2); }
+        }
+      }
+      // heaviest first
+      std::vector<int> order(items.size());
+      for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
+      std::vector<dev::PivotItem> sorted(items.size());
+      for (size_t i = 0; i < order.size(); i++) sorted[i] = items[order[i]];
+      sample_items(&sorted, t.sample);
+      void* d = nullptr;
+      HIP_TRY(hipMalloc(&d, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
+      if (!sorted.empty())
+        HIP_TRY(hipMemcpy(d, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice));
+      it = g->piv_items.emplace(key, std::make_pair(d, (int)sorted.size())).first;
     }
+    d_items = (const dev::PivotItem*)it->second.first;
+    nitems = it->second.second;
   }
-  // heaviest first
-  std::vector<int> order(items.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return icost[a] > icost[b]; });
-  std::vector<dev::PivotItem> sorted(items.size());
-  for (size_t i = 0; i < order.size(); i++) sorted[i] = items[order[i]];
-  sample_items(&sorted, t.sample);
 
-  const int grid = std::max(1, std::min<int>(pivot_grid(t), (int)sorted.size()));
+  const int grid = std::max(1, std::min<int>(pivot_grid(t), nitems));
   const long long dstride = ((long long)g->n + 63) / 64 * 64;
   // jobs linear in the triple region take a negative part (nj more accumulators)
   rc = plan_accumulators(lp, nj, stream, err, lin);
@@ -1174,20 +1196,14 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   DevScratch& sc = *lp->scratch;
   unsigned* dbuf = nullptr;
   JobDesc* d_jobs = nullptr;
-  dev::PivotItem* d_items = nullptr;
   HIP_TRY(sc.get(&dbuf, sizeof(unsigned) * dstride * grid));
   const int waves = pivot_waves(t, need_c3);
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(sc.get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
-  HIP_TRY(sc.get(&d_items, sizeof(dev::PivotItem) * std::max<size_t>(1, sorted.size())));
   // D starts at zero; every launch leaves it zero again (phase C of the kernel)
   HIP_TRY(hipMemsetAsync(dbuf, 0, sizeof(unsigned) * dstride * grid, stream));
   if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
-  if (!sorted.empty())
-    HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(dev::PivotItem) * sorted.size(), hipMemcpyHostToDevice,
-                           stream));
-  HIP_TRY(hipStreamSynchronize(stream));  // the host arrays go out of scope
-  const int nitems = (int)sorted.size();
+  HIP_TRY(hipStreamSynchronize(stream));  // the host job array goes out of scope
   const int c3 = need_c3 ? 1 : 0;
   bool full = false;  // any fringe vector with a multi-vertex anchor set
   for (auto& j : jobs)
