@@ -528,8 +528,7 @@ def main():
                 units["centre_pass_vertices"], units["centre_pass_positions"] = n, m
                 model += 8 * (n + m)
             roofline["alg_model"] = {"units": units, "bytes_per_unit": "20 paths_d + 4 paths_hub + 16 marks_d (20 in a "
-                                     "launch without c3) + 4 marks_hub + 12 xvisits + 4 xvisit_bits + 16 centres + "
-                                     "4 centre_bits + (16+s) c3_x + (8+s) c3_search_bits + s list_elems + 16 walk_d + "
+                                     "launch without c3) + 4 marks_hub + 12 xvisits + 20 centres + (16+s) c3_x + (8+s) c3_search_bits + s list_elems + 16 walk_d + "
                                      "12 walk_hub + 8 centre_pass_vertices + 8 centre_pass_positions, s = list element "
                                      "bytes", "list_elem_bytes": 2 if maxdeg <= 65536 else 4, "bytes": model,
                                      "equals_kernel": int(r.alg_bytes) == model,

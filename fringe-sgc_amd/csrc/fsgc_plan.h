@@ -60,6 +60,7 @@ struct DeviceGraph {
   int32_t* tsup = nullptr;      // m: |N(u) & N(x)| per directed edge
   int32_t* adjdeg = nullptr;    // m: degree of adj[e] (read in CSR order instead of through idx)
   int32_t* adjhub = nullptr;    // m: LDS slot of adj[e] in the pivot kernel when it is a hub, else ~adj[e]
+  int32_t* rev = nullptr;       // m: CSR index of the reverse edge (x, u) of (u, x) (built with the lists)
   long long* tl_off = nullptr;  // m + 1: offsets of the common-neighbour lists
   int32_t* tl = nullptr;        // positions in N(u) of the common neighbours of (u, x)
   uint16_t* tl16 = nullptr;     // the same lists in 16 bits (built when maxdeg <= 65536; pivot kernel)
@@ -70,6 +71,8 @@ struct DeviceGraph {
   std::vector<long long> h_piv_cost;
   // pivot-schedule work items on the device, per (chunk << 32 | sample): pointer, count
   std::map<long long, std::pair<void*, int>> piv_items;
+  // rank-schedule work items, the same way
+  std::map<long long, std::pair<void*, int>> tri_items;
   size_t derived_bytes = 0;              // device bytes of the derived arrays
 };
 

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(kComBlock) void k_common(const int* __restrict__ id
                                                       const int* __restrict__ adjdeg,
                                                       const ComItem* __restrict__ items, int nitems,
                                                       int* __restrict__ head, int* __restrict__ tsup,
-                                                      const long long* __restrict__ tl_off, int* __restrict__ tl)
+                                                      const long long* __restrict__ tl_off, int* __restrict__ tl,
+                                                      int* __restrict__ rev)
 {
   __shared__ int s_stage[kComWaves][kComStage];
   // per wave, one entry per lane's x: offset of its list in the packed stream (entry 64
@@ -188,6 +189,11 @@ __global__ __launch_bounds__(kComBlock) void k_common(const int* __restrict__ id
       sxb[lane] = xb;
       cnt[lane] = 0;
       if constexpr (kList) {
+        // the reverse-edge index: the CSR index of (x, u) for (u, x) and back
+        if (take) {
+          rev[ub + p] = exu;
+          rev[exu] = ub + p;
+        }
         s_bux[wave][lane] = take ? tl_off[ub + p] : 0;
         s_bxu[wave][lane] = take ? tl_off[exu] : 0;
         s_lim[wave][lane] = take ? tsup[ub + p] : 0;
@@ -436,6 +442,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
                                                            const int* __restrict__ tsup,
                                                            const int* __restrict__ adjdeg,
                                                            const int* __restrict__ adjhub,
+                                                           const int* __restrict__ rev,
                                                            const long long* __restrict__ tl_off,
                                                            const TL* __restrict__ tl, const JobDesc* __restrict__ jobs,
                                                            int njobs, int need_c3, const PivotItem* __restrict__ items,
@@ -573,7 +580,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       const int i = wb + c;
       if (i >= we) break;
       const int x = adj[i];
-      const int xb = g.beg(x), xe = g.end(x);
+      const int xe = g.end(x);
       if (lane == 0 && x > w) {
         const int hx = kHub > 0 ? adjhub[i] : ~x;
         if (kHub > 0 && hx >= 0) {
@@ -585,8 +592,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           bytes += kHub > 0 ? 20 : 16;
         }
       }
-      const int s = lower_bound_pos(adj, xb, xe, w + 1);
-      if (lane == 0) bytes += 12 + 4 * (32 - __clz(xe - xb + 1));
+      const int s = rev[i] + 1;  // past w in N(x)
+      if (lane == 0) bytes += 12;
       const int nch = (xe - s + achunk - 1) / achunk;
       if (nch > 1) {
         int slot = 0;
@@ -614,8 +621,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
           int h, k;
           heavy_task(t, nh, h, k);
           const int x = adj[s_heavy[h]];
-          const int xb = g.beg(x), xe = g.end(x);
-          const int s = lower_bound_pos(adj, xb, xe, w + 1) + k * achunk;
+          const int xe = g.end(x);
+          const int s = rev[s_heavy[h]] + 1 + k * achunk;
           scatter2(s, min(xe, s + achunk));
         }
         __syncthreads();
@@ -797,12 +804,12 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
       const int u = adj[wb + p];
       const int ub = g.beg(u), ue = g.end(u), du = ue - ub;
       const int tuw = tsup[wb + p];
-      const int rq = lower_bound_pos(adj, ub, ue, w) - ub;  // position of w in N(u)
+      const int rq = rev[wb + p] - ub;  // position of w in N(u)
       if (rq + 1 >= du) continue;
       bool in_slice = true;
       if (jobs[0].smod > 1) in_slice = (u % jobs[0].smod) == jobs[0].srem;
       if (!in_slice) continue;
-      if (lane == 0) bytes += 16 + 4 * (32 - __clz(du + 1));
+      if (lane == 0) bytes += 20;
       const bool c3on = need_c3 && tuw > 0;
       const int cstep = c3on ? ecap : wchunk;
       const int nwin = (du - (rq + 1) + cstep - 1) / cstep;
@@ -842,7 +849,7 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         const int u = adj[wb + p];
         const int ub = g.beg(u), ue = g.end(u), du = ue - ub;
         const int tuw = tsup[wb + p];
-        const int rq = lower_bound_pos(adj, ub, ue, w) - ub;
+        const int rq = rev[wb + p] - ub;
         const bool c3on = need_c3 && tuw > 0;
         const int cstep = c3on ? ecap : wchunk;
         const long long tb = c3on ? tl_off[ub + rq] : 0, te = c3on ? tl_off[ub + rq + 1] : 0;
@@ -916,6 +923,8 @@ void device_derived_free(DeviceGraph* g)
   if (g->tsup) (void)hipFree(g->tsup);
   if (g->adjdeg) (void)hipFree(g->adjdeg);
   if (g->adjhub) (void)hipFree(g->adjhub);
+  if (g->rev) (void)hipFree(g->rev);
+  g->rev = nullptr;
   if (g->tl_off) (void)hipFree(g->tl_off);
   if (g->tl) (void)hipFree(g->tl);
   if (g->tl16) (void)hipFree(g->tl16);
@@ -924,6 +933,8 @@ void device_derived_free(DeviceGraph* g)
   g->c3tl = nullptr;
   for (auto& kv : g->piv_items) (void)hipFree(kv.second.first);
   g->piv_items.clear();
+  for (auto& kv : g->tri_items) (void)hipFree(kv.second.first);
+  g->tri_items.clear();
   g->tsup = nullptr;
   g->adjdeg = nullptr;
   g->adjhub = nullptr;
@@ -1014,7 +1025,8 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     }
     if (common_items()) { if (err) *err = "upload of the intersection work items failed"; return -5; }
     hipLaunchKernelGGL(dev::k_common<false>, dim3(cgrid), dim3(dev::kComBlock), 0, stream, g->idx, g->adj,
-                       g->adjdeg, d_citems, ncitems, d_head, g->tsup, (const long long*)nullptr, (int*)nullptr);
+                       g->adjdeg, d_citems, ncitems, d_head, g->tsup, (const long long*)nullptr, (int*)nullptr,
+                       (int*)nullptr);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(dev::k_pivot_cost, dim3((n + 255) / 256), dim3(256), 0, stream, g->idx, g->adj, n, g->piv_cost);
     HIP_TRY(hipGetLastError());
@@ -1039,12 +1051,14 @@ int device_derived_build(DeviceGraph* g, void* stream_v, double* ms, std::string
     HIP_TRY(hipStreamSynchronize(stream));
     g->tl_total = total;
     HIP_TRY(hipMalloc((void**)&g->tl, sizeof(int) * std::max<long long>(1, total)));
+    HIP_TRY(hipMalloc((void**)&g->rev, sizeof(int) * std::max<long long>(1, m)));
     HIP_TRY(hipMemsetAsync(d_head + 1, 0, sizeof(int), stream));
     if (common_items()) { if (err) *err = "upload of the intersection work items failed"; return -5; }
     hipLaunchKernelGGL(dev::k_common<true>, dim3(cgrid), dim3(dev::kComBlock), 0, stream, g->idx, g->adj,
-                       g->adjdeg, d_citems, ncitems, d_head + 1, g->tsup, (const long long*)g->tl_off, g->tl);
+                       g->adjdeg, d_citems, ncitems, d_head + 1, g->tsup, (const long long*)g->tl_off, g->tl,
+                       g->rev);
     HIP_TRY(hipGetLastError());
-    g->derived_bytes += sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total;
+    g->derived_bytes += sizeof(long long) * ((size_t)m + 1) + sizeof(int) * (size_t)total + sizeof(int) * (size_t)m;
     // 16-bit copy of the lists for the pivot kernel: a position in N(u) is below
     // d_u <= maxdeg, so it fits when maxdeg <= 65536
     if (g->maxdeg <= 65536 && total > 0) {
@@ -1216,6 +1230,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   const int* tsup = g->tsup;
   const int* adjdeg = g->adjdeg;
   const int* adjhub = g->adjhub;
+  const int* rev = g->rev;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   const uint16_t* tl16 = g->tl16;
@@ -1253,7 +1268,8 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_PIVOT_LAUNCH(NJ, FULL, W, CAP, HUB, TL, TLP)                                                          \
   hipLaunchKernelGGL((dev::k_wedge_pivot<NJ, FULL, W, CAP, HUB, TL>), dim3(grid), dim3(W * dev::kWave), 0, s, idx, \
-                     adj, tsup, adjdeg, adjhub, tl_off, TLP, d_jobs, nj, c3, d_items + first, count, head, dbuf,   \
+                     adj, tsup, adjdeg, adjhub, rev, tl_off, TLP, d_jobs, nj, c3, d_items + first, count, head,   \
+                     dbuf,                                                                                        \
                      dstride, std::min(CAP, ecap), achunk, wchunk)
 #define FSGC_PIVOT_C3(NJ, FULL, W, CAP)                                   \
   do {                                                                    \

@@ -114,6 +114,7 @@ template <int NJ, bool kFull>
 __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_triangle_rank(const int* __restrict__ idx, const int* __restrict__ adj,
                                                              const int* __restrict__ tsup,
                                                              const int* __restrict__ adjdeg,
+                                                             const int* __restrict__ rev,
                                                              const long long* __restrict__ tl_off,
                                                              const int* __restrict__ tl,
                                                              const JobDesc* __restrict__ jobs, int njobs, int need_c3,
@@ -159,8 +160,8 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
       // T(y, x) lists the same common neighbours as T(x, y), in the same (id) order,
       // as positions in N(y): entry k of one gives entry k of the other
       const int yb2 = g.beg(y);
-      const long long yxb = tl_off[yb2 + lbound(adj, yb2, g.end(y), x) - yb2];
-      if (lane == 0) bytes += 4 + 12 + 4 * (32 - __clz(dy + 1)) + (need_c3 ? (ye - yb) * (lds ? 8 : 10) : 0);
+      const long long yxb = tl_off[rev[xb + py]];  // the reverse edge (y, x)
+      if (lane == 0) bytes += 4 + 4 + 8 + (need_c3 ? (ye - yb) * (lds ? 8 : 10) : 0);
       // mark T(x, y)
       if (need_c3) {
         for (long long k = yb + lane; k < ye; k += kWave) M[tl[k]] = 1;
@@ -355,7 +356,7 @@ static int triangle_grid(const Tune& t)
 size_t triangle_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
   const size_t mstride = ((size_t)g.maxdeg + 63) / 64 * 64;
-  return mstride * (size_t)triangle_grid(t) * dev::kTriWaves + sizeof(dev::TriItem) * ((size_t)g.n + 1024);
+  return mstride * (size_t)triangle_grid(t) * dev::kTriWaves;  // the work items stay with the graph
 }
 
 size_t edgepass_scratch_bytes(const DeviceGraph& g, const Tune& t)
@@ -546,40 +547,55 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
       for (int r = 0; r < j.rot; r++) full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
     }
   }
-  // items: vertices x with their positions split into bounded pieces
-  std::vector<dev::TriItem> items;
-  const long long budget = std::max<long long>(64, t.chunk);
-  for (int x = 0; x < g->n; x++) {
-    const int dx = g->h_idx[x + 1] - g->h_idx[x];
-    if (dx < 2) continue;
-    const int per = (int)std::max<long long>(1, budget * 64 / std::max(64, dx));
-    for (int p = 0; p < dx; p += per) items.push_back(dev::TriItem{x, p, std::min(dx, p + per), 0});
+  // items: vertices x with their positions split into bounded pieces; they depend on the
+  // graph, the chunk and the sample only, so they are built once per graph and key and
+  // kept on the device until the graph is freed
+  const dev::TriItem* d_items = nullptr;
+  int nitems = 0;
+  {
+    static std::mutex items_mu;
+    std::lock_guard<std::mutex> lk(items_mu);
+    const long long key = ((long long)t.chunk << 32) | (unsigned)std::max(1, t.sample);
+    auto it = g->tri_items.find(key);
+    if (it == g->tri_items.end()) {
+      std::vector<dev::TriItem> items;
+      const long long budget = std::max<long long>(64, t.chunk);
+      for (int x = 0; x < g->n; x++) {
+        const int dx = g->h_idx[x + 1] - g->h_idx[x];
+        if (dx < 2) continue;
+        const int per = (int)std::max<long long>(1, budget * 64 / std::max(64, dx));
+        for (int p = 0; p < dx; p += per) items.push_back(dev::TriItem{x, p, std::min(dx, p + per), 0});
+      }
+      sample_items(&items, t.sample);
+      void* d = nullptr;
+      HIP_TRY(hipMalloc(&d, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
+      if (!items.empty())
+        HIP_TRY(hipMemcpy(d, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice));
+      it = g->tri_items.emplace(key, std::make_pair(d, (int)items.size())).first;
+    }
+    d_items = (const dev::TriItem*)it->second.first;
+    nitems = it->second.second;
   }
-  sample_items(&items, t.sample);
   int grid = triangle_grid(t);
-  grid = std::max(1, std::min<int>(grid, ((int)items.size() + dev::kTriWaves - 1) / dev::kTriWaves));
+  grid = std::max(1, std::min<int>(grid, (nitems + dev::kTriWaves - 1) / dev::kTriWaves));
   const long long mstride = ((long long)g->maxdeg + 63) / 64 * 64;
   rc = plan_accumulators(lp, nj, stream, err);
   if (rc) return rc;
   unsigned char* gmark = nullptr;
   JobDesc* d_jobs = nullptr;
-  dev::TriItem* d_items = nullptr;
   HIP_TRY(lp->scratch->get(&gmark, (size_t)mstride * grid * dev::kTriWaves));
   for (int k = 0; k < nj; k++) jobs[k].out = lp->d_acc + k;
   HIP_TRY(lp->scratch->get(&d_jobs, sizeof(JobDesc) * std::max(1, nj)));
-  HIP_TRY(lp->scratch->get(&d_items, sizeof(dev::TriItem) * std::max<size_t>(1, items.size())));
   // marks start at zero; every launch unmarks what it marked
   HIP_TRY(hipMemsetAsync(gmark, 0, (size_t)mstride * grid * dev::kTriWaves, stream));
   if (nj) HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, stream));
-  if (!items.empty())
-    HIP_TRY(hipMemcpyAsync(d_items, items.data(), sizeof(dev::TriItem) * items.size(), hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipStreamSynchronize(stream));  // the host arrays go out of scope
-  const int nitems = (int)items.size();
+  HIP_TRY(hipStreamSynchronize(stream));  // the host job array goes out of scope
   if (nitems == 0) return 0;
   const int* idx = g->idx;
   const int* adj = g->adj;
   const int* tsup = g->tsup;
   const int* adjdeg = g->adjdeg;
+  const int* rev = g->rev;
   const long long* tl_off = g->tl_off;
   const int* tl = g->tl;
   const int c3 = need_c3 ? 1 : 0, lt = need_lt ? 1 : 0, tl_arg = std::max(0, tails);
@@ -591,7 +607,7 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
     hipStream_t s = (hipStream_t)sv;
 #define FSGC_TRI_LAUNCH(NJ, FULL)                                                                               \
   hipLaunchKernelGGL((dev::k_triangle_rank<NJ, FULL>), dim3(grid), dim3(dev::kTriBlock), 0, s, idx, adj, tsup,  \
-                     adjdeg, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride, \
+                     adjdeg, rev, tl_off, tl, d_jobs, nj, c3, lt, tl_arg, d_items + first, count, head, gmark, mstride, \
                      markcap, nullptr)
     if (nj == 1) {
       if (full) FSGC_TRI_LAUNCH(1, true); else FSGC_TRI_LAUNCH(1, false);
@@ -659,7 +675,7 @@ int device_c3tl_build(DeviceGraph* g, void* stream_v, double* ms, std::string* e
     if (!items.empty()) {
       hipLaunchKernelGGL((dev::k_triangle_rank<1, true>), dim3(grid), dim3(dev::kTriBlock), 0, stream,
                          (const int*)g->idx, (const int*)g->adj, (const int*)g->tsup, (const int*)g->adjdeg,
-                         (const long long*)g->tl_off, (const int*)g->tl, (const JobDesc*)d_job, 0, 1, 0, 0,
+                         (const int*)g->rev, (const long long*)g->tl_off, (const int*)g->tl, (const JobDesc*)d_job, 0, 1, 0, 0,
                          (const dev::TriItem*)d_items, (int)items.size(), head, gmark, mstride, dev::kMarkCap, c3);
       HIP_TRY(hipGetLastError());
     }

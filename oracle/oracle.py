@@ -139,7 +139,8 @@ def pivot_split_budget(total_cost: int, chunk: int = 256) -> int:
 
 
 def pivot_units(nindex, nlist, ecap: int, hub_slots: int, c3: bool, chunk: int = 256, threads: int = 0) -> dict:
-    """Unit counts of one pivot-schedule launch on this CSR (oracle.cpp oracle_pivot_units)."""
+    """Unit counts of one pivot-schedule launch on this CSR (oracle.cpp oracle_pivot_units).
+    hub_slots: the LDS hub counters of the launch (0 for launches with c3)."""
     ni = np.ascontiguousarray(nindex, dtype=np.int32)
     nl = np.ascontiguousarray(nlist, dtype=np.int32)
     deg = np.diff(ni).astype(np.int64)
@@ -156,7 +157,10 @@ def pivot_bytes(units: dict, list_elem_bytes: int, hub_launch: bool) -> int:
     """The pivot kernel's algorithmic bytes from its units (the per-unit sizes of
     pivot.hip k_wedge_pivot)."""
     s = list_elem_bytes
+    # a pivot neighbour x reads adj, idx[x + 1] and the reverse-edge index (12 B); a centre
+    # adj, idx twice, the support and the reverse-edge index (20 B): no searches (the
+    # *_bits units are the probes the searches they replace would take)
     return (20 * units["paths_d"] + 4 * units["paths_hub"] + (20 if hub_launch else 16) * units["marks_d"]
-            + 4 * units["marks_hub"] + 12 * units["xvisits"] + 4 * units["xvisit_bits"] + 16 * units["centres"]
-            + 4 * units["centre_bits"] + (16 + s) * units["c3_x"] + (8 + s) * units["c3_search_bits"]
+            + 4 * units["marks_hub"] + 12 * units["xvisits"] + 20 * units["centres"]
+            + (16 + s) * units["c3_x"] + (8 + s) * units["c3_search_bits"]
             + s * units["list_elems"] + 16 * units["walk_d"] + 12 * units["walk_hub"])

@@ -154,13 +154,14 @@ def test_triangle_rank_schedule_matches_per_root(gpu):
 
 
 def test_wedge_pivot_hub_and_global_counters(gpu):
-    """Graphs with more vertices than the pivot kernel has hub slots (39808). Launches
-    without c3 (the closed-form wedge kinds) then keep the 2-hop counters of the
-    highest-degree vertices in LDS and the others in global memory; launches with c3,
-    and the 4- and 8-wave shapes, keep all of them in global memory. Each shape, and a
+    """Graphs with more vertices than the pivot kernel has LDS hub counters (39552).
+    Launches without c3 (the closed-form wedge kinds and the motifs linear in the
+    triple region) then keep the 2-hop counters of the highest-degree vertices in LDS
+    and the others in global memory; launches with c3, and the 4-wave shape, keep all
+    of them in global memory. Each shape (12-wave default, 16, 8 and 4 waves), and a
     batch that mixes c3 and closed-form motifs, matches the oracle and the per-root
     schedule."""
-    for spec in ("er:50000:200000:41", "chunglu:50000:2.1:8:500:43"):
+    for spec in ("er:120000:400000:41", "chunglu:120000:2.1:8:500:43"):
         ni, nl = gpu.generate_csr(spec)
         g = gpu.Graph.from_csr(ni, nl)
         mos = [gpu.preprocess(*C2_MOTIF)[0], gpu.preprocess_pattern(13)[0], gpu.preprocess_pattern(15)[0]]
@@ -172,10 +173,40 @@ def test_wedge_pivot_hub_and_global_counters(gpu):
             same(a, o, f"pivot {gpu.live_fields(mo)} on {spec}")
             b = g.count(mo, gpu.Tune(schedule=1))
             assert (b.sg, b.core_matches) == (a.sg, a.core_matches)
-            for block in (256, 512):
+            for block in (256, 512, 1024):
                 c = g.count(mo, gpu.Tune(block=block))
                 assert (c.exact, c.sg, c.core_matches) == (a.exact, a.sg, a.core_matches), block
             refs.append(a)
         batch, _ = g.count_batch(mos)
         for a, b in zip(refs, batch):
             assert (a.exact, a.sg, a.core_matches, a.flags) == (b.exact, b.sg, b.core_matches, b.flags)
+
+
+def test_wedge_pivot_large_pivot_degree(gpu):
+    """A pivot of degree 33000 (a star on vertex 0 plus random edges, on the generator's
+    ids): its items are split over several workgroups and its 2-hop counts reach the
+    thousands. The default hub shape, the 4-wave shape (global counters only) and the
+    per-root schedule agree on every field."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    n, hub = 40000, 33000
+    pairs = {(0, i) for i in range(1, hub + 1)}
+    while len(pairs) < hub + 120000:
+        a, b = (int(x) for x in rng.integers(1, n, 2))
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    e = np.array(sorted(pairs), dtype=np.int64)
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    ni = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(ni, src + 1, 1)
+    ni = np.cumsum(ni).astype(np.int32)
+    g = gpu.Graph.from_csr(ni, dst.astype(np.int32))
+    for mo in (gpu.preprocess(*C2_MOTIF)[0], gpu.preprocess_pattern(13)[0], gpu.preprocess_pattern(15)[0]):
+        a = g.count(mo)
+        for t in (gpu.Tune(block=256), gpu.Tune(schedule=1)):
+            b = g.count(mo, t)
+            assert (a.exact, a.sg, a.core_matches, a.flags) == (b.exact, b.sg, b.core_matches, b.flags), gpu.live_fields(mo)
+    g.free()
