@@ -348,9 +348,12 @@ __device__ __noinline__ u128 wedge_lin_tri(int f, int du, int dv, int dw, int tu
               wedge_lin_v(f, dw - 2, du - 2, dv - 2, tuw - 1, cvw - 1, tuv - 1, fl), fl);
 }
 
+// cb: the centre's binomials C(d_u - 2 - c, f), c = 0..2, when they are inline u64
+// values (cb_ok, the same for the whole walk of a centre), for wedges that are not
+// triangles
 __device__ __forceinline__ bool eval_wedge_lin(const JobDesc* __restrict__ J, int u, int w, int du, int dv, int dw,
                                                int tuv, int tuw, int cvw, bool tri, u128& val, bool& match,
-                                               uint32_t& fl)
+                                               uint32_t& fl, bool cb_ok, ull cb0, ull cb1, ull cb2)
 {
   const int* p = J->p;
   if (tri && !(u < w)) return false;
@@ -359,8 +362,17 @@ __device__ __forceinline__ bool eval_wedge_lin(const JobDesc* __restrict__ J, in
               dw >= max(max(sum1, sum3 - du - dv), max(sum2 - du, sum2 - dv));
   if (pass && !tri) pass = du >= min_u && dv >= lim && dw >= max(lim, sum2 - dv);
   uint32_t lf = 0;
-  if (!tri) val = wedge_lin_v(f, du - 2, dw - 1, dv - 1, tuw, tuv, cvw - 1, lf);
-  else val = wedge_lin_tri(f, du, dv, dw, tuv, tuw, cvw, lf);
+  if (!tri && cb_ok) {
+    const int a = dw - 1, b = dv - 1, p = tuw, q = tuv, r = cvw - 1;
+    const ull t0 = (ull)((long long)(a - p) * (b - q) - r + a + b);
+    const ull t1 = (ull)((long long)p * (b - q) + (long long)q * (a - p));
+    const ull t2 = (ull)((long long)p * q);
+    val = (u128)cb0 * t0 + (u128)cb1 * t1 + (u128)cb2 * t2;  // as wedge_lin_v's inline case
+  } else if (!tri) {
+    val = wedge_lin_v(f, du - 2, dw - 1, dv - 1, tuw, tuv, cvw - 1, lf);
+  } else {
+    val = wedge_lin_tri(f, du, dv, dw, tuv, tuw, cvw, lf);
+  }
   fl |= pass ? lf : (lf & kOvf128);
   match = pass;
   return true;
@@ -636,6 +648,23 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     auto window = [&](int u, int ub, int du, int tuw, bool c3on, long long tb, long long te, int q_lo, int q_hi) {
       unsigned* __restrict__ E = s_e[wave];
       const bool last = q_hi == du;
+      // the centre's binomials of the linear jobs (uniform over the wave)
+      bool cb_ok[NJ];
+      ull cb0[NJ], cb1[NJ], cb2[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        cb_ok[j] = false;
+        cb0[j] = cb1[j] = cb2[j] = 0;
+        if (!(kPivWaves == 16 && kHub == 0) && j < njobs && jobs[j].p[11]) {
+          const int f = jobs[j].p[12], nx = du - 2;
+          cb_ok[j] = fc3_small_inline_ok(f, nx);
+          if (cb_ok[j]) {
+            cb0[j] = ubin_small(nx, f);
+            cb1[j] = ubin_small(nx - 1, f);
+            cb2[j] = ubin_small(nx - 2, f);
+          }
+        }
+      }
       if (c3on) {
         for (int q = lane; q < q_hi - q_lo; q += kWave) E[q] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -769,7 +798,8 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
               // the 16-wave c3 shape never runs a linear job (a launch with c3 evaluates
               // them with their c3); the others may run launches without c3
               if (!(kPivWaves == 16 && kHub == 0) && jobs[j].p[11]) {
-                take = eval_wedge_lin(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, tri, val, match, acc[j].fl);
+                take = eval_wedge_lin(jobs + j, u, w, du, dv, dw, tuv, tuw, cvw, tri, val, match, acc[j].fl, cb_ok[j],
+                                      cb0[j], cb1[j], cb2[j]);
               } else if constexpr (kHub > 0) {
                 // the hub shape runs launches without c3: besides the linear jobs only
                 // specialised ones (any other general job makes the launch take c3)
