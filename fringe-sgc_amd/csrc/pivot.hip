@@ -561,17 +561,17 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
     // With hub slots, adjhub[e] names the LDS counter of a hub (>= 0) or ~y otherwise.
     // Waves take the x one at a time; an x with more than kAChunk positions past w is
     // deferred (phase A2) and its chunks shared by all waves.
-    // Four list loads per lane are issued before their (fire-and-forget) atomics.
+    // Eight list loads per lane are issued before their (fire-and-forget) atomics.
     auto scatter2 = [&](int k0, int k1) {
-      for (int b = k0; b < k1; b += 4 * kWave) {
-        int hv[4];
+      for (int b = k0; b < k1; b += 8 * kWave) {
+        int hv[8];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < 8; r++) {
           const int k = b + r * kWave + lane;
           hv[r] = k < k1 ? (kHub > 0 ? adjhub[k] : ~adj[k]) : 0;
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < 8; r++) {
           if (b + r * kWave + lane >= k1) continue;
           const int h = hv[r];
           if (kHub > 0 && h >= 0) {
@@ -740,51 +740,29 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      // walk the window, software-pipelined: the loads of a position run two iterations
-      // (128 positions) ahead of its evaluation and its 2-hop counter load one ahead, so
-      // a wave's random D reads overlap the evaluation instead of stalling every step
-      int h1 = 0, t1 = 0, d1 = 0;  // position + 64: loaded
-      int t0 = 0, d0 = 0;          // position: loaded, counter loaded
-      unsigned r0 = 0u;
-      bool g0 = false;             // its counter is in D (not an LDS hub slot)
-      {
-        const int q = q_lo + lane;
-        int h0 = 0;
+      // walk the window, software-pipelined in steps of P * 64 positions: the list loads
+      // of a step run two steps ahead of its evaluation and its 2-hop counter loads one
+      // step ahead, so each wait for memory is covered by the evaluation of a step
+      auto pos_load = [&](int q, int& h, int& t, int& d) __attribute__((always_inline)) {
         if (q < q_hi) {
-          h0 = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
-          t0 = tsup[ub + q];
-          d0 = adjdeg[ub + q];
-          g0 = !(kHub > 0 && h0 >= 0);
-          r0 = g0 ? D[~h0] : s_hub[kHub > 0 ? h0 : 0];
+          h = kHub > 0 ? adjhub[ub + q] : ~adj[ub + q];
+          t = tsup[ub + q];
+          d = adjdeg[ub + q];
         }
-        if (q + kWave < q_hi) {
-          h1 = kHub > 0 ? adjhub[ub + q + kWave] : ~adj[ub + q + kWave];
-          t1 = tsup[ub + q + kWave];
-          d1 = adjdeg[ub + q + kWave];
-        }
-      }
-      for (int q0 = q_lo; q0 < q_hi; q0 += kWave) {
-        const int q = q0 + lane;
-        // the counter of position q + 64, then the loads of q + 128
-        unsigned r1 = 0u;
-        bool g1 = false;
-        if (q + kWave < q_hi) {
-          g1 = !(kHub > 0 && h1 >= 0);
-          r1 = g1 ? D[~h1] : s_hub[kHub > 0 ? h1 : 0];
-        }
-        int h2 = 0, t2 = 0, d2 = 0;
-        if (q + 2 * kWave < q_hi) {
-          h2 = kHub > 0 ? adjhub[ub + q + 2 * kWave] : ~adj[ub + q + 2 * kWave];
-          t2 = tsup[ub + q + 2 * kWave];
-          d2 = adjdeg[ub + q + 2 * kWave];
-        }
+      };
+      auto cnt_load = [&](int q, int h, unsigned& r, bool& gd) __attribute__((always_inline)) {
         if (q < q_hi) {
-          unsigned dv_ = r0;
-          if (g0) {
-            dv_ = (r0 >> 24) == epoch ? (r0 & 0xffffffu) : 0u;
+          gd = !(kHub > 0 && h >= 0);  // its counter is in D, not an LDS hub slot
+          r = gd ? D[~h] : s_hub[kHub > 0 ? h : 0];
+        }
+      };
+      auto eval_pos = [&](int q, int tuv, int dv, unsigned r, bool gd) __attribute__((always_inline)) {
+        if (q < q_hi) {
+          unsigned dv_ = r;
+          if (gd) {
+            dv_ = (r >> 24) == epoch ? (r & 0xffffffu) : 0u;
             bytes += 4;
           }
-          const int tuv = t0, dv = d0;
           const bool tri = dv_ & 1u;
           const int cvw = (int)(dv_ >> 1);
           const int c3 = c3on ? (int)E[q - q_lo] : 0;
@@ -814,8 +792,51 @@ __global__ __launch_bounds__(kPivWaves * kWave) void k_wedge_pivot(const int* __
             }
           }
         }
-        r0 = r1; g0 = g1; t0 = t1; d0 = d1;
-        h1 = h2; t1 = t2; d1 = d2;
+      };
+      // P positions per lane per step: two in the 12- and 8-wave hub shapes, one where
+      // the registers of a second would spill (16 waves, and the c3 shapes, which
+      // measured 3% slower with two)
+      constexpr int P = (kHub > 0 && kPivWaves < 16) ? 2 : 1;
+      constexpr int kStep = P * kWave;
+      const int qa = q_lo + lane;
+      int t0[P], d0[P];  // this step: loaded, counters loaded
+      unsigned r0[P];
+      bool g0[P];
+      int h1[P], t1[P], d1[P];  // the next step: loaded
+#pragma unroll
+      for (int z = 0; z < P; z++) {
+        int h0 = 0;
+        t0[z] = d0[z] = 0;
+        r0[z] = 0u;
+        g0[z] = false;
+        h1[z] = t1[z] = d1[z] = 0;
+        pos_load(qa + z * kWave, h0, t0[z], d0[z]);
+        cnt_load(qa + z * kWave, h0, r0[z], g0[z]);
+        pos_load(qa + kStep + z * kWave, h1[z], t1[z], d1[z]);
+      }
+      for (int q0 = q_lo; q0 < q_hi; q0 += kStep) {
+        const int q = q0 + lane;
+        unsigned r1[P];
+        bool g1[P];
+        int h2[P], t2[P], d2[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) {
+          r1[z] = 0u;
+          g1[z] = false;
+          cnt_load(q + kStep + z * kWave, h1[z], r1[z], g1[z]);
+        }
+#pragma unroll
+        for (int z = 0; z < P; z++) {
+          h2[z] = t2[z] = d2[z] = 0;
+          pos_load(q + 2 * kStep + z * kWave, h2[z], t2[z], d2[z]);
+        }
+#pragma unroll
+        for (int z = 0; z < P; z++) eval_pos(q + z * kWave, t0[z], d0[z], r0[z], g0[z]);
+#pragma unroll
+        for (int z = 0; z < P; z++) {
+          t0[z] = t1[z]; d0[z] = d1[z]; r0[z] = r1[z]; g0[z] = g1[z];
+          h1[z] = h2[z]; t1[z] = t2[z]; d1[z] = d2[z];
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
