@@ -210,8 +210,8 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
         if (need_c3) {
           const bool big = kFull && act && !need_lt && (ze - zb) > kCoopLen;
           if (act && !big) {
-            for (long long q = zb; q < ze; q++) {
-              const int pq = tl[q];
+            // four list loads per lane in flight, then their marks
+            auto mark_hit = [&](int pq) __attribute__((always_inline)) {
               if (M[pq]) {
                 c3++;
                 if (need_lt && pq < wpos) {
@@ -219,7 +219,16 @@ __global__ __launch_bounds__(kTriBlock) __attribute__((amdgpu_waves_per_eu(4))) 
                   if (tails) wq = addx(wq, ncr(adjdeg[xb + pq] - 3, tails, fl0), fl0);
                 }
               }
+            };
+            long long q = zb;
+            for (; q + 4 <= ze; q += 4) {
+              int pq[4];
+#pragma unroll
+              for (int r = 0; r < 4; r++) pq[r] = tl[q + r];
+#pragma unroll
+              for (int r = 0; r < 4; r++) mark_hit(pq[r]);
             }
+            for (; q < ze; q++) mark_hit(tl[q]);
           }
           ull bm = __ballot(big);
           while (bm) {
