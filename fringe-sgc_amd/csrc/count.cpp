@@ -153,7 +153,8 @@ bool needs_full(const JobDesc& j)
 {
   if (j.cls != kWedge && j.cls != kTriangle) return false;
   bool multi = false;
-  for (int r = 0; r < j.rot; r++) multi = multi || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+  // fringes anchored on a pair of core vertices (those on all three have an inline form)
+  for (int r = 0; r < j.rot; r++) multi = multi || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6]) != 0;
   return multi;
 }
 

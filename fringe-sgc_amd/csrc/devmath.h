@@ -461,6 +461,24 @@ __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, u
 
 __device__ __forceinline__ bool fc3_is_singleton(const int* fr) { return (fr[iuv] | fr[iuw] | fr[ivw] | fr[iuvw]) == 0; }
 
+// Single-vertex anchor sets plus k = fr[iuvw] fringes anchored on all three core
+// vertices: those can only take vertices of the triple region, so fc3 =
+// C(s, k) * (the single-vertex count with the triple region reduced by k), which is
+// fc3_peeled's one placement when no pair anchor set has fringes (the same binomials,
+// so the same flags), inline. Regions non-negative.
+__device__ __forceinline__ u128 fc3_singletons_uvw(const int* fr, const int* venn, uint32_t& fl)
+{
+  const int k = fr[iuvw];
+  if (k == 0) return fc3_singletons(fr, venn, fl);
+  if (venn[iuvw] < k) return 0;
+  int v[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) v[r] = venn[r];
+  const u128 c = binom(v[iuvw], k, fl);
+  v[iuvw] -= k;
+  return mulx(c, fc3_singletons(fr, v, fl), fl);
+}
+
 __device__ __forceinline__ bool venn_nonneg(const int* venn)
 {
   return (venn[iu] | venn[iv] | venn[iw] | venn[iuv] | venn[iuw] | venn[ivw] | venn[iuvw]) >= 0;

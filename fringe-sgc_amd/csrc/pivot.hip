@@ -273,9 +273,9 @@ __device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t
     if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
     if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
-  // single-vertex anchor sets only (the regions are non-negative for every wedge
-  // and triangle that passes the filters)
-  return fc3_singletons(fr, venn, fl);
+  // single-vertex anchor sets, and fringes anchored on all three core vertices (the
+  // regions are non-negative for every wedge and triangle that passes the filters)
+  return fc3_singletons_uvw(fr, venn, fl);
 }
 
 __device__ __forceinline__ bool eval_wedge_special(const JobDesc* __restrict__ J, int du, int dv, int dw, int cvw,
@@ -1274,7 +1274,7 @@ int plan_wedge_pivot(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, 
   for (auto& j : jobs)
     if (j.kind == kWedgeGeneral)
       for (int r = 0; r < j.rot; r++)
-        full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+        full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6]) != 0;  // pair anchor sets
   if (nitems == 0) return 0;
   const int* idx = g->idx;
   const int* adj = g->adj;

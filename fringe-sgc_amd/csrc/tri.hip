@@ -58,9 +58,9 @@ __device__ __forceinline__ u128 fc3_tri(const int* fr, const int* venn, uint32_t
     if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
     if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
   }
-  // single-vertex anchor sets only (the regions are non-negative for every wedge
-  // and triangle that passes the filters)
-  return fc3_singletons(fr, venn, fl);
+  // single-vertex anchor sets, and fringes anchored on all three core vertices (the
+  // regions are non-negative for every wedge and triangle that passes the filters)
+  return fc3_singletons_uvw(fr, venn, fl);
 }
 
 // One triangle in reference roles (u > v > w by id) for one job. `c3lt` is the number
@@ -553,7 +553,9 @@ int plan_triangles(std::vector<JobDesc>& jobs, DeviceGraph* g, const Tune& t, vo
     if (j.cls == kTriangle) {
       need_c3 = true;
       if (j.kind != kTriGeneral) need_lt = true;
-      for (int r = 0; r < j.rot; r++) full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6] | j.fr[r][7]) != 0;
+      // fringes anchored on a pair of core vertices need the full fc3; single-vertex and
+      // all-three anchor sets have the inline form
+      for (int r = 0; r < j.rot; r++) full = full || (j.fr[r][3] | j.fr[r][5] | j.fr[r][6]) != 0;
     }
   }
   // items: vertices x with their positions split into bounded pieces; they depend on the
