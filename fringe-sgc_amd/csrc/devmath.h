@@ -461,6 +461,30 @@ __device__ __forceinline__ u128 fc3_singletons(const int* fr, const int* venn, u
 
 __device__ __forceinline__ bool fc3_is_singleton(const int* fr) { return (fr[iuv] | fr[iuw] | fr[ivw] | fr[iuvw]) == 0; }
 
+// fc3 when exactly one pair anchor set ps (iuv, iuw or ivw) carries fringes, besides
+// single-vertex and all-three anchor sets: fc3_peeled's loops reduce to the placements
+// of its f fringes, i in the triple region and f - i in the pair's own region, each
+// times the single-vertex count on what remains. The same binomials in the same cases
+// (so the same value and flags), inline. Regions non-negative.
+__device__ __forceinline__ u128 fc3_one_pair(const int* fr, const int* venn, int ps, uint32_t& fl)
+{
+  int v[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) v[r] = venn[r];
+  if (v[iuvw] < fr[iuvw]) return 0;
+  const u128 cw = binom(v[iuvw], fr[iuvw], fl);
+  const int s0 = v[iuvw] - fr[iuvw], p0 = v[ps], fp = fr[ps];
+  u128 total = 0;
+  for (int i = 0; i <= min(s0, fp); i++) {
+    if (p0 < fp - i) continue;
+    const u128 c = mulx(mulx(cw, binom(s0, i, fl), fl), binom(p0, fp - i, fl), fl);
+    v[iuvw] = s0 - i;
+    v[ps] = p0 - (fp - i);
+    total = addx(total, mulx(c, fc3_singletons(fr, v, fl), fl), fl);
+  }
+  return total;
+}
+
 // Single-vertex anchor sets plus k = fr[iuvw] fringes anchored on all three core
 // vertices: those can only take vertices of the triple region, so fc3 =
 // C(s, k) * (the single-vertex count with the triple region reduced by k), which is

@@ -271,7 +271,12 @@ __device__ __forceinline__ u128 fc3_any(const int* fr, const int* venn, uint32_t
 {
   if constexpr (kFull) {
     if (!venn_nonneg(venn)) return fc3(fr, venn, fl);  // the reference's loop, its conventions
-    if (!fc3_is_singleton(fr)) return fc3_peeled(fr, venn, fl);
+    if (!fc3_is_singleton(fr)) {
+      // one pair anchor set: the inline placement sum; several: the peeled loops
+      const int np = (fr[iuv] != 0) + (fr[iuw] != 0) + (fr[ivw] != 0);
+      if (np == 1) return fc3_one_pair(fr, venn, fr[iuv] ? iuv : (fr[iuw] ? iuw : ivw), fl);
+      if (np > 1) return fc3_peeled(fr, venn, fl);
+    }
   }
   // single-vertex anchor sets, and fringes anchored on all three core vertices (the
   // regions are non-negative for every wedge and triangle that passes the filters)
