@@ -546,6 +546,21 @@ def main():
         served = {k: sv[k] for k in ("graphs", "offered_rate_qps", "queries", "slots", "p50_ms", "p99_ms",
                                      "motifs_per_s", "identical_alone_vs_served")}
 
+    # the full-size batched (C3: R-MAT s22) and served (C5: R-MAT s23 + s21) measurements,
+    # as committed under profiles/ by their own runs (minutes each; not re-run here)
+    records = {}
+    for key, name, fields in (("c3_batched", "batch_c3_rmat22_54.json",
+                               ("graph", "motifs", "excluded", "plain_s", "batched_s", "speedup", "identical")),
+                              ("c5_served", "serve_c5_graphs_all_fast_classes_0.03qps.json",
+                               ("graphs", "classes", "offered_rate_qps", "queries", "slots", "p50_ms", "p99_ms",
+                                "motifs_per_s", "identical_alone_vs_served"))):
+        path = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(path):
+            rec = json.load(open(path))
+            rec = rec.get("summary", rec)
+            records[key] = {f: rec.get(f) for f in fields}
+            records[key]["source"] = f"profiles/{name} (committed record, not re-run by this invocation)"
+
     out = {
         "metric": "core_matches_per_s",
         "value": round(value, 1),
@@ -578,6 +593,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "full_size_records": records or None,
     }
     print(json.dumps(out), flush=True)
 
