@@ -776,6 +776,10 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
     if (d == 0) continue;
     int64_t per = t.chunk;
     if (heavy && d > 64) per = std::max<int64_t>(1, (int64_t)t.chunk * 64 / d);
+    // a general core's root edge (u, v) extends over about d_u * d further core matches,
+    // each with its own Venn vector: cut by d^2, so that a hub root spreads over many
+    // waves (at least 4 root edges per item)
+    if (j.cls == kGeneral) per = std::max<int64_t>(4, std::min<int64_t>(per, (int64_t)t.chunk * 64 / (d * d)));
     if (d > per || d > t.hub_degree) {
       emit();
       for (int64_t s = b; s < e; s += per) out->push_back(WorkItem{jid, u, u, 0, s, std::min(e, s + per)});
@@ -791,8 +795,9 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
 
 size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
-  // work items: at most one per `chunk` CSR positions plus one per split hub piece
-  const size_t per = (size_t)std::max(1, std::min(t.chunk, 64));
+  // work items: at most one per `chunk` CSR positions (per 4 for general cores) plus one
+  // per split hub piece
+  const size_t per = (size_t)std::max(1, std::min(t.chunk, 4));
   return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 64) + 4096;
 }
 
