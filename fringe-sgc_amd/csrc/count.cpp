@@ -125,7 +125,7 @@ Tune tune_for(fsgc_graph* g, const fsgc_tune* t, int sched)
   if (!t) return r;
   if (t->block > 0) r.block = t->block;
   if (t->grid > 0) r.grid = t->grid;
-  if (t->chunk > 0) r.chunk = t->chunk;
+  if (t->chunk > 0) { r.chunk = t->chunk; r.chunk_set = true; }
   if (t->hub_degree > 0) r.hub_degree = t->hub_degree;
   if (t->lds_cap > 0) r.lds_cap = t->lds_cap;
   return r;

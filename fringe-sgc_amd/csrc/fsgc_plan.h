@@ -30,6 +30,8 @@ struct Tune {
   int block = 0;          // threads per workgroup for kernels with several shapes (0: the kernel's default)
   int grid = 0;           // workgroups (0: 8 per CU)
   int chunk = 256;        // roots (CSR positions or vertices) per work item
+  bool chunk_set = false; // chunk given by the caller or the tuner (else the schedule's default: 256,
+                          // 64 for rank launches that count triple intersections)
   int hub_degree = 4096;  // roots of degree above this are split into single-root items
   int lds_cap = 0;        // > 0: cap on the positions staged in LDS per window / chunk / mark array (0: the kernel's maximum)
   int sample = 1;         // > 1: run only every sample-th work item (timing runs; partial counts)

@@ -85,7 +85,7 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
     std::memset(&def, 0, sizeof(def));
     def.block = s == fsgc::kPivot ? 0 : 256;  // the pivot kernel's default depends on the launch (c3 or not)
     def.grid = s == fsgc::kPivot ? cus : (s == fsgc::kRank ? cus * 4 : (s == fsgc::kClique ? cus * 3 : cus * 8));
-    def.chunk = d.chunk;
+    def.chunk = 0;  // each schedule's default item size
     def.hub_degree = d.hub_degree;
     def.lds_cap = d.lds_cap;
     double ms = 0, def_ms = 0;
@@ -119,7 +119,8 @@ int fsgc_autotune(fsgc_graph* g, const uint8_t* mos, int k, int32_t sample, fsgc
       fsgc::Tune& b = g->best[s];
       b.block = best.block;
       b.grid = best.grid;
-      b.chunk = best.chunk;
+      b.chunk = best.chunk > 0 ? best.chunk : d.chunk;
+      b.chunk_set = best.chunk > 0;
       b.hub_degree = best.hub_degree;
       b.lds_cap = best.lds_cap;
       b.sample = 1;
