@@ -17,12 +17,17 @@
 //     centres take several windows; each x's list is cut to the window by two
 //     binary searches); the cut lists of a wave's 64 x's are streamed as one
 //     sequence, so short and long lists alike keep eight loads per lane in flight;
-//   * a wave then walks the window and evaluates every wedge job per lane;
+//   * a wave then walks the window and evaluates every wedge job per lane; the walk
+//     is software-pipelined (list loads two steps ahead, 2-hop counter loads one);
+//   * the positions of w in N(x) and N(u) come from the per-graph reverse-edge index;
 //   * within a workgroup the waves share the work of one pivot through LDS task
 //     lists: neighbours x with long lists (2-hop scatter) and centres u with several
 //     windows (c3 scatter and walk) are cut into chunks that any wave takes.
 // The sum over wedges is the reference's, with the same per-wedge filters, formulas
-// and counted matches; only the order of evaluation differs.
+// and counted matches; only the order of evaluation differs. Jobs whose count is
+// linear in the triple count (f fringes on the centre, one on each endpoint) skip c3:
+// the kernel sums the rest of their formula per wedge and k_wedge_lin_centres adds the
+// c3 term per centre (eval_wedge_lin).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
