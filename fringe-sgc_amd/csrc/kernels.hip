@@ -336,7 +336,9 @@ __device__ void do_triangle(const JobDesc* __restrict__ J, const WorkItem& it, L
 
 // FC:345-405 (commonX): venn[S] = number of non-core vertices whose set of adjacent
 // anchors is exactly S. Each vertex is counted once, at its lowest adjacent anchor.
-__device__ void coop_commonx(const G& g, int nodes, int nonanch, const int* st, int* venn)
+// With sub_core = false the core vertices are left in (the prefix counts of
+// coop_leaf_venn).
+__device__ void coop_commonx(const G& g, int nodes, int nonanch, const int* st, int* venn, bool sub_core = true)
 {
   venn[lane_id()] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -360,8 +362,59 @@ __device__ void coop_commonx(const G& g, int nodes, int nonanch, const int* st, 
       atomicAdd(&venn[bmp], 1);
     }
   }
-  if (lane_id() < nodes) {
+  if (sub_core && lane_id() < nodes) {
     const int x = st[lane_id()];
+    int bmp = 0;
+    for (int k = 0; k < nodes; k++) {
+      if ((nonanch >> k) & 1) continue;
+      if (contains(g.adj, g.beg(st[k]), g.end(st[k]), x)) bmp |= 1 << k;
+    }
+    if (bmp) atomicSub(&venn[bmp], 1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// coop_commonx of a full match from the counts P of its prefix (core vertices
+// 0 .. nodes - 2, core vertices left in) and the last core vertex x: only N(x) is
+// classified, by its prefix pattern p into cnt[p]; then venn[S | x] = cnt[S] and
+// venn[S] = P[S] - cnt[S] for S within the prefix (S != 0), and the core vertices are
+// subtracted as in coop_commonx. When x anchors no fringe, venn = P minus the core.
+__device__ void coop_leaf_venn(const G& g, int nodes, int nonanch, const int* st, const int* P, int* cnt, int* venn)
+{
+  const int lane = lane_id();
+  const int leaf = nodes - 1, lb = 1 << leaf;
+  const bool leaf_anch = !((nonanch >> leaf) & 1);
+  if (leaf_anch) {
+    cnt[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int a = st[leaf];
+    for (int e = g.beg(a) + lane; e < g.end(a); e += kWave) {
+      const int y = g.adj[e];
+      int pm = 0;
+      for (int k = 0; k < leaf; k++) {
+        if ((nonanch >> k) & 1) continue;
+        if (contains(g.adj, g.beg(st[k]), g.end(st[k]), y)) pm |= 1 << k;
+      }
+      atomicAdd(&cnt[pm], 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // lane S: the region of anchor set S (nodes <= 6, so every region has a lane)
+  int v = 0;
+  if (lane != 0 && lane < (1 << nodes)) {
+    if (lane & lb) v = leaf_anch ? cnt[lane & ~lb] : 0;
+    else v = P[lane] - (leaf_anch ? cnt[lane] : 0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  venn[lane] = v;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < nodes) {
+    const int x = st[lane];
     int bmp = 0;
     for (int k = 0; k < nodes; k++) {
       if ((nonanch >> k) & 1) continue;
@@ -436,6 +489,10 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
   const int wn = st.v[__builtin_ctz(list)];
   const int b = g.beg(wn), e = g.end(wn);
   const bool leaf = (L == nodes - 2);
+  // the prefix counts of coop_leaf_venn, computed on the first match of this prefix
+  int* P = slots + kVennSlots * 64 + 512;
+  int* cnt = P + 64;
+  bool have_p = false;
   for (int cb = b; cb < e; cb += kWave) {
     const int i = cb + lane_id();
     int x = -1;
@@ -462,7 +519,11 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
           hm &= hm - 1;
           Stack6 s2 = st;
           s2.v[L + 1] = bcast(x, w2);
-          coop_commonx(g, nodes, nonanch, s2.v, slots + slot * 64);
+          if (!have_p) {
+            coop_commonx(g, nodes - 1, nonanch, st.v, P, false);
+            have_p = true;
+          }
+          coop_leaf_venn(g, nodes, nonanch, s2.v, P, cnt, slots + slot * 64);
           if (lane_id() == w2) myslot = slot;
           slot++;
         }
@@ -521,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const JobDesc* __restrict__ jo
                                                   int nitems, int* __restrict__ head)
 {
   // per wave: kVennSlots staged Venn vectors, then 128 u128 of region-DP state
-  __shared__ __attribute__((aligned(16))) int s_venn[kWavesPerBlock][kVennSlots * 64 + 128 * 4];
+  __shared__ __attribute__((aligned(16))) int s_venn[kWavesPerBlock][kVennSlots * 64 + 128 * 4 + 128];
   int* slots = s_venn[threadIdx.x / kWave];
   LaneAcc acc;
   for (;;) {
