@@ -130,7 +130,9 @@ struct WorkItem {
   int32_t job;
   int32_t u0;     // owner of position `begin` (edge-rooted classes)
   int32_t u1;     // owner of position `end - 1`
-  int32_t pad;
+  // general cores: 0, or (parts << 8) | part for an item that holds one part of its
+  // root edges' first-level candidates (those at list offset i with i % parts == part)
+  int32_t split;
   int64_t begin;
   int64_t end;
 };

@@ -480,7 +480,8 @@ struct Stack6 { int v[6]; };
 
 // FC:489-564 (general_core_level<L>)
 template <int L>
-__device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, LaneAcc& acc, int* slots)
+__device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, LaneAcc& acc, int* slots,
+                          int parts = 0, int part = 0)
 {
   const int nodes = J->p[0];
   const int list = J->p[7 + L];
@@ -497,6 +498,8 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
     const int i = cb + lane_id();
     int x = -1;
     bool a = i < e;
+    // an item that holds one part of the first level's candidates (WorkItem::split)
+    if constexpr (L == 1) a = a && (parts <= 1 || (i - b) % parts == part);
     if (a) {
       x = g.adj[i];
       a = g.deg(x) >= mindeg;
@@ -562,6 +565,7 @@ __device__ void do_general(const JobDesc* __restrict__ J, const WorkItem& it, La
   const G g{J->idx, J->adj};
   const int startdeg = J->p[1], dd0 = J->p[2];
   const int smod = J->smod, srem = J->srem, n = J->n;
+  const int parts = it.split >> 8, part = it.split & 255;
   for (long long base = it.begin; base < it.end; base += kWave) {
     const Root r = load_root(g, it, base);
     const bool act = r.ok && r.du >= startdeg && r.dv >= dd0 && in_slice(r.u, smod, srem);
@@ -573,7 +577,7 @@ __device__ void do_general(const JobDesc* __restrict__ J, const WorkItem& it, La
       for (int q = 0; q < 6; q++) st.v[q] = 0;
       st.v[0] = bcast(r.u, who);
       st.v[1] = bcast(r.v, who);
-      gen_level<1>(J, g, st, acc, slots);
+      gen_level<1>(J, g, st, acc, slots, parts, part);
     }
   }
 }
@@ -814,6 +818,9 @@ void device_graph_free(DeviceGraph* dg)
 // Cuts the roots of one job into work items. Edge-rooted classes: consecutive CSR
 // positions, at most `chunk` per item, and for vertices whose per-position work grows
 // with their degree, fewer positions per item (roughly chunk * 64 / degree).
+// a general core's launch is cut into about this many items of equal estimated work
+constexpr int64_t kGeneralItems = 65536;
+
 static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tune& t, std::vector<WorkItem>* out)
 {
   if (j.cls == kVertex) {
@@ -824,6 +831,19 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
   }
   const bool heavy = (j.cls == kWedge || j.cls == kGeneral);
   const std::vector<int32_t>& idx = g.h_idx;
+  // a general core's root edge (u, v) extends over about d_u^2 further core matches, each
+  // with its own Venn vector: items hold about `wt` such units, so that the longest item
+  // is a small share of the launch (a hub root edge's first-level candidates are split
+  // over up to 64 items)
+  int64_t wt = 0;
+  if (j.cls == kGeneral) {
+    double total = 0;
+    for (int32_t u = 0; u < g.n; u++) {
+      const double d = idx[u + 1] - idx[u];
+      total += d * d * d;
+    }
+    wt = std::max<int64_t>((int64_t)t.chunk * 4, (int64_t)(total / kGeneralItems));
+  }
   int64_t cur_b = -1, cur_n = 0;
   int32_t cur_u0 = 0, cur_u1 = 0;
   auto emit = [&]() {
@@ -837,13 +857,19 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
     if (d == 0) continue;
     int64_t per = t.chunk;
     if (heavy && d > 64) per = std::max<int64_t>(1, (int64_t)t.chunk * 64 / d);
-    // a general core's root edge (u, v) extends over about d_u * d further core matches,
-    // each with its own Venn vector: cut by d^2, so that a hub root spreads over many
-    // waves (at least 4 root edges per item)
-    if (j.cls == kGeneral) per = std::max<int64_t>(4, std::min<int64_t>(per, (int64_t)t.chunk * 64 / (d * d)));
+    int parts = 1;
+    if (j.cls == kGeneral) {
+      if (d * d <= wt) per = std::max<int64_t>(1, std::min<int64_t>(per, wt / (d * d)));
+      else {
+        per = 1;
+        parts = (int)std::min<int64_t>(std::min<int64_t>(64, d), (d * d + wt - 1) / wt);
+      }
+    }
     if (d > per || d > t.hub_degree) {
       emit();
-      for (int64_t s = b; s < e; s += per) out->push_back(WorkItem{jid, u, u, 0, s, std::min(e, s + per)});
+      for (int64_t s = b; s < e; s += per)
+        for (int q = 0; q < parts; q++)
+          out->push_back(WorkItem{jid, u, u, parts > 1 ? (parts << 8) | q : 0, s, std::min(e, s + per)});
       continue;
     }
     if (cur_b < 0) { cur_b = b; cur_u0 = u; }
@@ -856,10 +882,11 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
 
 size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
-  // work items: at most one per `chunk` CSR positions (per 4 for general cores) plus one
-  // per split hub piece
+  // work items: at most one per `chunk` CSR positions (per 4) plus one per split hub
+  // piece; a general core's hub items, cut by work, add at most about
+  // 4 * kGeneralItems
   const size_t per = (size_t)std::max(1, std::min(t.chunk, 4));
-  return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 64) + 4096;
+  return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 4 * (size_t)kGeneralItems + 64) + 4096;
 }
 
 int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,
