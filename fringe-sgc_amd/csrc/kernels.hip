@@ -426,41 +426,127 @@ __device__ void coop_leaf_venn(const G& g, int nodes, int nonanch, const int* st
   __builtin_amdgcn_wave_barrier();
 }
 
-// The fringe count of one general-core match by the region DP of the job's tables,
-// wave-cooperative: lane t holds state t. `venn` is the match's Venn vector and `tab`
-// 128 u128 of LDS per wave. Returns the same value as functionx on every lane; a
-// negative region that an anchor set may draw from makes it 0, as the reference's
-// nCk(n < 0, k) does.
-__device__ u128 functionx_dp(const JobDesc* __restrict__ J, const int* venn, u128* tab, uint32_t& fl)
+// LDS staging of a prefix's anchor lists: at most kPrefixCap ints per wave, after
+// kPrefixMeta ints of ranges
+constexpr int kPrefixCap = 2048;
+constexpr int kPrefixMeta = 16;
+
+// Copies the neighbour lists of the prefix's anchored core vertices (0 .. pn - 1) to
+// `lst` when together they fit kPrefixCap ints: list k at [meta[2k], meta[2k + 1]).
+// Lane c < pn gets in `ppc` the prefix anchors adjacent to core vertex c.
+// Wave-cooperative, uniform arguments; false (nothing staged) when the lists do not fit.
+__device__ bool stage_prefix(const G& g, int pn, int nonanch, const int* st, int* meta, int* lst, int& ppc)
+{
+  int tot = 0;
+  for (int k = 0; k < pn; k++)
+    if (!((nonanch >> k) & 1)) tot += g.deg(st[k]);
+  if (tot > kPrefixCap) return false;
+  int off = 0;
+  for (int k = 0; k < pn; k++) {
+    int b = 0, d = 0;
+    if (!((nonanch >> k) & 1)) { b = g.beg(st[k]); d = g.end(st[k]) - b; }
+    for (int i = lane_id(); i < d; i += kWave) lst[off + i] = g.adj[b + i];
+    if (lane_id() == 0) { meta[2 * k] = off; meta[2 * k + 1] = off + d; }
+    off += d;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  ppc = 0;
+  if (lane_id() < pn) {
+    const int c = st[lane_id()];
+    for (int k = 0; k < pn; k++)
+      if (!((nonanch >> k) & 1) && contains(lst, meta[2 * k], meta[2 * k + 1], c)) ppc |= 1 << k;
+  }
+  return true;
+}
+
+// coop_leaf_venn with the prefix's anchor lists staged in LDS (stage_prefix): N(x) is
+// classified by LDS searches, and x's adjacency to the prefix's core vertices (for the
+// core subtraction) is read off the same walk; ppc (lane c) is core vertex c's
+// prefix pattern. Same result as coop_leaf_venn.
+__device__ void coop_leaf_venn_lds(const G& g, int nodes, int nonanch, const int* st, const int* P, int* cnt,
+                                   int* venn, const int* meta, const int* lst, int ppc)
+{
+  const int lane = lane_id();
+  const int leaf = nodes - 1, lb = 1 << leaf;
+  const bool leaf_anch = !((nonanch >> leaf) & 1);
+  const int pmask = ~nonanch & (lb - 1);
+  const int x = st[leaf];
+  int adjx = 0;  // bit k: x adjacent to core vertex k (k < leaf)
+  if (leaf_anch) {
+    cnt[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int mine = 0;
+    for (int e = g.beg(x) + lane; e < g.end(x); e += kWave) {
+      const int y = g.adj[e];
+      int pm = 0;
+      for (int k = 0; k < leaf; k++) {
+        if (y == st[k]) mine |= 1 << k;
+        if (((pmask >> k) & 1) && contains(lst, meta[2 * k], meta[2 * k + 1], y)) pm |= 1 << k;
+      }
+      atomicAdd(&cnt[pm], 1);
+    }
+    for (int k = 0; k < leaf; k++)
+      if (__ballot((mine >> k) & 1)) adjx |= 1 << k;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    for (int k = 0; k < leaf; k++)
+      if (((pmask >> k) & 1) && contains(lst, meta[2 * k], meta[2 * k + 1], x)) adjx |= 1 << k;
+  }
+  int v = 0;
+  if (lane != 0 && lane < (1 << nodes)) {
+    if (lane & lb) v = leaf_anch ? cnt[lane & ~lb] : 0;
+    else v = P[lane] - (leaf_anch ? cnt[lane] : 0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  venn[lane] = v;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // each core vertex leaves the region of its adjacent anchors
+  int bmp = 0;
+  if (lane < leaf) bmp = ppc | ((leaf_anch && ((adjx >> lane) & 1)) ? lb : 0);
+  else if (lane == leaf) bmp = adjx & pmask;
+  if (bmp) atomicSub(&venn[bmp], 1);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ ull dp_mul(ull a, ull b, uint32_t&) { return a * b; }
+__device__ __forceinline__ ull dp_add(ull a, ull b, uint32_t&) { return a + b; }
+__device__ __forceinline__ u128 dp_mul(u128 a, u128 b, uint32_t& fl) { return mulx(a, b, fl); }
+__device__ __forceinline__ u128 dp_add(u128 a, u128 b, uint32_t& fl) { return addx(a, b, fl); }
+
+// The region DP of functionx_dp in words of type T (ull only where no value can wrap):
+// `old` and `bin` are 64 T of LDS each.
+template <typename T>
+__device__ __noinline__ T region_dp(const JobDesc* __restrict__ J, const int* venn, T* old, T* bin, uint32_t& fl)
 {
   const int lane = lane_id();
   const int ns = J->dp_ns, nreg = J->dp_nreg;
-  bool neg = false;
-  for (int r = 0; r < nreg; r++) neg = neg || venn[J->dp_reg[r]] < 0;
-  if (neg) return 0;
   constexpr ull kGuard = 0x8080808080808080ull;
   const ull dig_t = lane < ns ? J->dp_dig[lane] : 0;
-  u128 cur = lane == 0 ? (u128)1 : (u128)0;
-  u128* old = tab;
-  u128* bin = tab + 64;
+  T cur = lane == 0 ? (T)1 : (T)0;
   for (int r = 0; r < nreg; r++) {
     const int nr = venn[J->dp_reg[r]];
     if (nr == 0) continue;
     const int maxj = J->dp_maxj[r];
     old[lane] = cur;
-    bin[lane] = lane <= maxj ? ncr(nr, lane, fl) : (u128)0;
+    bin[lane] = lane <= maxj ? (T)ncr(nr, lane, fl) : (T)0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const ull allow = J->dp_allow[r];
-    u128 acc = 0;
+    T acc = 0;
     for (int d = 0; d < ns; d++) {
       const ull dd = J->dp_dig[d];
       if (dd & ~allow) continue;
       if (J->dp_sum[d] > maxj) continue;
       if (lane < ns && (((dig_t | kGuard) - dd) & kGuard) == kGuard) {
-        const u128 coef = mulx(bin[J->dp_sum[d]], (u128)J->dp_mult[d], fl);
-        acc = addx(acc, mulx(old[lane - d], coef, fl), fl);
+        const T coef = dp_mul(bin[J->dp_sum[d]], (T)J->dp_mult[d], fl);
+        acc = dp_add(acc, dp_mul(old[lane - d], coef, fl), fl);
       }
     }
     cur = acc;
@@ -471,9 +557,39 @@ __device__ u128 functionx_dp(const JobDesc* __restrict__ J, const int* venn, u12
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const u128 res = old[ns - 1];
+  const T res = old[ns - 1];
   __builtin_amdgcn_wave_barrier();
   return res;
+}
+
+// The fringe count of one general-core match by the region DP of the job's tables,
+// wave-cooperative: lane t holds state t. `venn` is the match's Venn vector and `tab`
+// 128 u128 of LDS per wave. Returns the same value as functionx on every lane; a
+// negative region that an anchor set may draw from makes it 0, as the reference's
+// nCk(n < 0, k) does. With N vertices in the regions drawn from and F fringes, every
+// DP value, binomial and product is at most N^F (a count of choices of at most F of
+// the N vertices), so the DP runs in u64 when N^F < 2^64; the binomials are the same
+// calls either way, so the reference-compatibility flags are too.
+__device__ u128 functionx_dp(const JobDesc* __restrict__ J, const int* venn, u128* tab, uint32_t& fl)
+{
+  const int ns = J->dp_ns, nreg = J->dp_nreg;
+  bool neg = false;
+  ull n_all = 0;
+  for (int r = 0; r < nreg; r++) {
+    const int nr = venn[J->dp_reg[r]];
+    neg = neg || nr < 0;
+    n_all += nr > 0 ? (ull)nr : 0;
+  }
+  if (neg) return 0;
+  const int nf = J->dp_sum[ns - 1];
+  bool fits = true;
+  ull pw = 1;
+  for (int i = 0; i < nf && fits; i++) {
+    if (__umul64hi(pw, n_all) != 0) fits = false;
+    else pw *= n_all;
+  }
+  if (fits) return (u128)region_dp<ull>(J, venn, (ull*)tab, (ull*)tab + 64, fl);
+  return region_dp<u128>(J, venn, tab, tab + 64, fl);
 }
 
 struct Stack6 { int v[6]; };
@@ -490,10 +606,14 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
   const int wn = st.v[__builtin_ctz(list)];
   const int b = g.beg(wn), e = g.end(wn);
   const bool leaf = (L == nodes - 2);
-  // the prefix counts of coop_leaf_venn, computed on the first match of this prefix
+  // the prefix counts of coop_leaf_venn and the staged prefix lists, set up on the
+  // first match of this prefix
   int* P = slots + kVennSlots * 64 + 512;
   int* cnt = P + 64;
-  bool have_p = false;
+  int* meta = cnt + 64;
+  int* lst = meta + kPrefixMeta;
+  bool have_p = false, staged = false;
+  int ppc = 0;
   for (int cb = b; cb < e; cb += kWave) {
     const int i = cb + lane_id();
     int x = -1;
@@ -524,9 +644,11 @@ __device__ void gen_level(const JobDesc* __restrict__ J, const G& g, Stack6 st, 
           s2.v[L + 1] = bcast(x, w2);
           if (!have_p) {
             coop_commonx(g, nodes - 1, nonanch, st.v, P, false);
+            staged = stage_prefix(g, nodes - 1, nonanch, st.v, meta, lst, ppc);
             have_p = true;
           }
-          coop_leaf_venn(g, nodes, nonanch, s2.v, P, cnt, slots + slot * 64);
+          if (staged) coop_leaf_venn_lds(g, nodes, nonanch, s2.v, P, cnt, slots + slot * 64, meta, lst, ppc);
+          else coop_leaf_venn(g, nodes, nonanch, s2.v, P, cnt, slots + slot * 64);
           if (lane_id() == w2) myslot = slot;
           slot++;
         }
@@ -585,8 +707,10 @@ __device__ void do_general(const JobDesc* __restrict__ J, const WorkItem& it, La
 __global__ __launch_bounds__(kBlock) void k_count(const JobDesc* __restrict__ jobs, const WorkItem* __restrict__ items,
                                                   int nitems, int* __restrict__ head)
 {
-  // per wave: kVennSlots staged Venn vectors, then 128 u128 of region-DP state
-  __shared__ __attribute__((aligned(16))) int s_venn[kWavesPerBlock][kVennSlots * 64 + 128 * 4 + 128];
+  // per wave: kVennSlots staged Venn vectors, 128 u128 of region-DP state, a prefix's
+  // counts and leaf histogram (64 + 64 ints), then its staged anchor lists
+  __shared__ __attribute__((aligned(16))) int s_venn[kWavesPerBlock][kVennSlots * 64 + 128 * 4 + 128 + kPrefixMeta +
+                                                                     kPrefixCap];
   int* slots = s_venn[threadIdx.x / kWave];
   LaneAcc acc;
   for (;;) {

@@ -51,3 +51,29 @@ def test_split_items_with_slices_and_batch(gpu):
         c = g.count(mos[31], gpu.Tune(slice_mod=5, slice_rem=rem, chunk=4096))
         assert key(s) == key(c), rem
     g.free()
+
+
+def path_motif(gpu, f0, f2, f3):
+    """Path core 0-1-2-3 with f0, f2 and f3 fringes on core vertices 0, 2 and 3."""
+    edges = [(0, 1), (1, 2), (2, 3)]
+    v = 4
+    for a, c in ((0, f0), (2, f2), (3, f3)):
+        for _ in range(c):
+            edges.append((a, v))
+            v += 1
+    return gpu.preprocess(v, edges)[0]
+
+
+@pytest.mark.parametrize("spec", ["rmat:6:6:0.57:0.19:0.19:5", "rmat:7:8:0.57:0.19:0.19:9", "er:30:120:3"])
+def test_region_dp_word_size(gpu, spec):
+    """The region DP runs in u64 for a match whose N regional vertices and F fringes give
+    N^F < 2^64, in u128 otherwise. F = 14 switches at N = 24, F = 11 at N = 57, F = 21 at
+    N = 9, F = 8 at N = 256: on these graphs (maximum degree 11 to about 60) one launch
+    mixes both word sizes. Results and flags equal the oracle's."""
+    ni, nl = gpu.generate_csr(spec)
+    g = gpu.Graph.from_csr(ni, nl)
+    for fr in ((12, 1, 1), (8, 2, 1), (20, 0, 1), (6, 1, 1)):
+        mo = path_motif(gpu, *fr)
+        print(spec, fr, flush=True)
+        same(g.count(mo), oracle.count(mo, ni, nl), f"{fr} on {spec}")
+    g.free()
