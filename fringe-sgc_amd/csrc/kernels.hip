@@ -943,14 +943,15 @@ void device_graph_free(DeviceGraph* dg)
 // positions, at most `chunk` per item, and for vertices whose per-position work grows
 // with their degree, fewer positions per item (roughly chunk * 64 / degree).
 // a general core's launch is cut into about this many items of equal estimated work
-constexpr int64_t kGeneralItems = 65536;
+constexpr int64_t kGeneralItems = 1 << 20;
 
-static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tune& t, std::vector<WorkItem>* out)
+template <class Emit>
+static void cut_items_with(const JobDesc& j, const DeviceGraph& g, int jid, const Tune& t, Emit&& put)
 {
   if (j.cls == kVertex) {
     if (j.kind != kVertexStars) return;
     const int64_t step = std::max<int64_t>(64, (int64_t)t.chunk * 16);
-    for (int64_t b = 0; b < g.n; b += step) out->push_back(WorkItem{jid, 0, 0, 0, b, std::min<int64_t>(g.n, b + step)});
+    for (int64_t b = 0; b < g.n; b += step) put(WorkItem{jid, 0, 0, 0, b, std::min<int64_t>(g.n, b + step)});
     return;
   }
   const bool heavy = (j.cls == kWedge || j.cls == kGeneral);
@@ -971,7 +972,7 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
   int64_t cur_b = -1, cur_n = 0;
   int32_t cur_u0 = 0, cur_u1 = 0;
   auto emit = [&]() {
-    if (cur_b >= 0) out->push_back(WorkItem{jid, cur_u0, cur_u1, 0, cur_b, idx[cur_u1 + 1]});
+    if (cur_b >= 0) put(WorkItem{jid, cur_u0, cur_u1, 0, cur_b, idx[cur_u1 + 1]});
     cur_b = -1;
     cur_n = 0;
   };
@@ -993,7 +994,7 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
       emit();
       for (int64_t s = b; s < e; s += per)
         for (int q = 0; q < parts; q++)
-          out->push_back(WorkItem{jid, u, u, parts > 1 ? (parts << 8) | q : 0, s, std::min(e, s + per)});
+          put(WorkItem{jid, u, u, parts > 1 ? (parts << 8) | q : 0, s, std::min(e, s + per)});
       continue;
     }
     if (cur_b < 0) { cur_b = b; cur_u0 = u; }
@@ -1004,13 +1005,21 @@ static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tun
   emit();
 }
 
+static void cut_items(const JobDesc& j, const DeviceGraph& g, int jid, const Tune& t, std::vector<WorkItem>* out)
+{
+  cut_items_with(j, g, jid, t, [out](const WorkItem& w) { out->push_back(w); });
+}
+
 size_t perroot_scratch_bytes(const DeviceGraph& g, const Tune& t)
 {
   // work items: at most one per `chunk` CSR positions (per 4) plus one per split hub
-  // piece; a general core's hub items, cut by work, add at most about
-  // 4 * kGeneralItems
+  // piece, or the items of a general core (cut by work), counted here
   const size_t per = (size_t)std::max(1, std::min(t.chunk, 4));
-  return sizeof(WorkItem) * ((size_t)g.m / per + (size_t)g.n + 4 * (size_t)kGeneralItems + 64) + 4096;
+  JobDesc gj{};
+  gj.cls = kGeneral;
+  size_t ngen = 0;
+  cut_items_with(gj, g, 0, t, [&ngen](const WorkItem&) { ngen++; });
+  return sizeof(WorkItem) * std::max((size_t)g.m / per + (size_t)g.n + 64, ngen) + 4096;
 }
 
 int plan_jobs(std::vector<JobDesc>& jobs, const std::vector<const DeviceGraph*>& graphs, const Tune& t,

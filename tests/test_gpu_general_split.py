@@ -34,8 +34,8 @@ def test_split_items_equal_oracle(gpu):
 
 def test_split_items_with_slices_and_batch(gpu):
     """Root slices and a batch of split general jobs beside other classes equal the
-    counts with few splits (R-MAT s9, hubs of degree up to 279: about 1239 units per
-    item by default, 63 parts per hub root edge; 16384 units and 5 parts at chunk 4096)."""
+    counts with few splits (R-MAT s9, hubs of degree up to 279: about 1024 units per
+    item by default, 64 parts per hub root edge; 16384 units and 5 parts at chunk 4096)."""
     g = gpu.Graph.generate("rmat:9:16:0.57:0.19:0.19:11")
     mos = gpu.random_motifs(3, 64)
     ids = (31, 6, 18, 32, 0, 1, 8)
